@@ -1,0 +1,316 @@
+/*
+ * gray_device.h -- per-ray device math of the GRay2 geodesic path for
+ * gfx950, templated on the arithmetic type T (double or float).
+ *
+ * The double instantiation is compiled with -ffp-contract=off and evaluates
+ * every expression in exactly the order written here, which is the order of
+ * the reference kernels (sim-org/KS.cl, phys.cl, RK4.cl, driver.cl) with
+ * OpenCL dot() taken left to right.  Division and sqrt in double are the
+ * correctly rounded IEEE operations, so the plain FP64 form reproduces a
+ * correctly rounded CPU evaluation of the same sequence bit for bit.
+ *
+ * The float instantiation is compiled with contraction on and uses the
+ * hardware reciprocal / square-root / exp2 instructions (about 1 ulp).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "grayhip_params.h"
+
+namespace gray {
+
+template <class T> struct Mth;
+
+template <> struct Mth<double> {
+	static __device__ __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
+	static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+	static __device__ __forceinline__ double exp_(double x) { return exp(x); }
+};
+
+template <> struct Mth<float> {
+	static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+	static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+	static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
+};
+
+template <class T> struct V4 { T t, x, y, z; };
+
+template <class T>
+__device__ __forceinline__ T dot4(const V4<T> &a, const V4<T> &b)
+{
+	return ((a.t * b.t + a.x * b.x) + a.y * b.y) + a.z * b.z;
+}
+
+/* sim-org/KS.cl:48-55 */
+template <class T>
+__device__ __forceinline__ T getrr(T a, const V4<T> &q)
+{
+	T aa = a * a;
+	T zz = q.z * q.z;
+	T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+	return Mth<T>::sqrt_(kk * kk + aa * zz) + kk;
+}
+
+/* sim-org/KS.cl:91-114: k_mu = g_{mu nu} u^nu, g = eta + f l l. */
+template <class T>
+__device__ __forceinline__ V4<T> down(T a, const V4<T> &q, const V4<T> &u)
+{
+	typedef Mth<T> M;
+	T aa = a * a;
+	T zz = q.z * q.z;
+	T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+	T rr = M::sqrt_(kk * kk + aa * zz) + kk;
+	T r  = M::sqrt_(rr);
+
+	T f  = M::div((T(2.0) * rr) * r, rr * rr + aa * zz);
+	T lx = M::div(r * q.x + a * q.y, rr + aa);
+	T ly = M::div(r * q.y - a * q.x, rr + aa);
+	T lz = M::div(q.z, r);
+
+	V4<T> gt = {T(-1.0) + f, f * lx, f * ly, f * lz};
+	V4<T> gx = {f * lx, T(1.0) + (f * lx) * lx, (f * lx) * ly, (f * lx) * lz};
+	V4<T> gy = {f * ly, (f * ly) * lx, T(1.0) + (f * ly) * ly, (f * ly) * lz};
+	V4<T> gz = {f * lz, (f * lz) * lx, (f * lz) * ly, T(1.0) + (f * lz) * lz};
+	V4<T> k = {dot4(gt, u), dot4(gx, u), dot4(gy, u), dot4(gz, u)};
+	return k;
+}
+
+template <class T>
+__device__ __forceinline__ T getuu(T a, const V4<T> &q, const V4<T> &u)
+{
+	return dot4(down(a, q, u), u);
+}
+
+/* sim-org/KS.cl:188-232 with the camera trigonometry passed in. */
+template <class T>
+__device__ __forceinline__ void gr_icond(T a, T r_obs, T si, T ci, T sj, T cj,
+                                         T alpha, T beta, V4<T> &q, V4<T> &u)
+{
+	typedef Mth<T> M;
+	T R0 = r_obs * si - beta * ci;
+	T z  = r_obs * ci + beta * si;
+	T y  = R0 * sj + alpha * cj;
+	T x  = R0 * cj - alpha * sj;
+	q = V4<T>{T(0.0), x, y, z};
+	u = V4<T>{T(1.0), si * cj, si * sj, ci};
+
+	T aa = a * a;
+	T zz = z * z;
+	T kk = T(0.5) * (((x * x + y * y) + zz) - aa);
+	T rr = M::sqrt_(kk * kk + aa * zz) + kk;
+	T r  = M::sqrt_(rr);
+	T f  = M::div((T(2.0) * rr) * r, rr * rr + aa * zz);
+	T lx = M::div(r * x + a * y, rr + aa);
+	T ly = M::div(r * y - a * x, rr + aa);
+	T lz = M::div(z, r);
+
+	T g01 = f * lx, g02 = f * ly, g03 = f * lz;
+	T g11 = T(1.0) + (f * lx) * lx, g12 = (f * lx) * ly, g13 = (f * lx) * lz;
+	T g21 = (f * ly) * lx, g22 = T(1.0) + (f * ly) * ly, g23 = (f * ly) * lz;
+	T g31 = (f * lz) * lx, g32 = (f * lz) * ly, g33 = T(1.0) + (f * lz) * lz;
+
+	T A = T(-1.0) + f;
+	T B = ((g01 * u.x + g02 * u.y) + g03 * u.z) * T(2.0);
+	T C = ((((g11 * u.x + g12 * u.y) + g13 * u.z) * u.x) +
+	       (((g21 * u.x + g22 * u.y) + g23 * u.z) * u.y)) +
+	       (((g31 * u.x + g32 * u.y) + g33 * u.z) * u.z);
+	T s = M::div(-(B + M::sqrt_(B * B - (T(4.0) * A) * C)), T(2.0) * A);
+	u.x = M::div(u.x, s);
+	u.y = M::div(u.y, s);
+	u.z = M::div(u.z, s);
+}
+
+/* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS.
+ * Returns du/dlambda (dq/dlambda = u). */
+template <class T>
+__device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u)
+{
+	typedef Mth<T> M;
+	T f, dx_f, dy_f, dz_f;
+	T lx, dx_lx, dy_lx, dz_lx;
+	T ly, dx_ly, dy_ly, dz_ly;
+	T lz, dx_lz, dy_lz, dz_lz;
+	T tmp;
+	{
+		T aa = a * a;
+		T zz = q.z * q.z;
+		T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+		T dd = M::sqrt_(kk * kk + aa * zz);
+		T rr = dd + kk;
+		T r  = M::sqrt_(rr);
+		T ir = M::div(T(1.0), r);
+		T ss = rr + aa;
+		T iss = M::div(T(1.0), ss);
+		T dx_r, dy_r, dz_r, tmp2;
+
+		tmp  = M::div(T(0.5), r * dd);
+		dz_r = (tmp * ss) * q.z;
+		tmp  = tmp * rr;
+		dy_r = tmp * q.y;
+		dx_r = tmp * q.x;
+		tmp  = M::div(T(2.0), rr + M::div(aa * zz, rr));
+
+		tmp2 = T(3.0) - (T(2.0) * rr) * tmp;
+		f    = tmp * r;
+		dx_f = (tmp * dx_r) * tmp2;
+		dy_f = (tmp * dy_r) * tmp2;
+		dz_f = tmp * (dz_r * tmp2 - ((tmp * aa) * q.z) * ir);
+
+		T m2r  = T(-2.0) * r;
+		T issr = iss * r;
+		T issa = iss * a;
+
+		lx    = iss * (q.x * r + q.y * a);
+		tmp   = iss * (q.x + m2r * lx);
+		dx_lx = tmp * dx_r + issr;
+		dy_lx = tmp * dy_r + issa;
+		dz_lx = tmp * dz_r;
+
+		ly    = iss * (q.y * r - q.x * a);
+		tmp   = iss * (q.y + m2r * ly);
+		dx_ly = tmp * dx_r - issa;
+		dy_ly = tmp * dy_r + issr;
+		dz_ly = tmp * dz_r;
+
+		lz    = q.z * ir;
+		tmp   = (-lz) * ir;
+		dx_lz = tmp * dx_r;
+		dy_lz = tmp * dy_r;
+		dz_lz = tmp * dz_r + ir;
+	}
+
+	T hDxu, hDyu, hDzu;
+	V4<T> uD;
+	{
+		V4<T> Dx, Dy, Dz;
+		T lu  = ((u.t + lx * u.x) + ly * u.y) + lz * u.z;
+		T flu = f * lu;
+		Dx.t = dx_f * lu + f * ((dx_lx * u.x + dx_ly * u.y) + dx_lz * u.z);
+		Dy.t = dy_f * lu + f * ((dy_lx * u.x + dy_ly * u.y) + dy_lz * u.z);
+		Dz.t = dz_f * lu + f * ((dz_lx * u.x + dz_ly * u.y) + dz_lz * u.z);
+
+		Dx.x = Dx.t * lx + flu * dx_lx;
+		Dx.y = Dx.t * ly + flu * dx_ly;
+		Dx.z = Dx.t * lz + flu * dx_lz;
+
+		Dy.x = Dy.t * lx + flu * dy_lx;
+		Dy.y = Dy.t * ly + flu * dy_ly;
+		Dy.z = Dy.t * lz + flu * dy_lz;
+
+		Dz.x = Dz.t * lx + flu * dz_lx;
+		Dz.y = Dz.t * ly + flu * dz_ly;
+		Dz.z = Dz.t * lz + flu * dz_lz;
+
+		hDxu = T(0.5) * dot4(Dx, u);
+		hDyu = T(0.5) * dot4(Dy, u);
+		hDzu = T(0.5) * dot4(Dz, u);
+
+		uD.t = (u.x * Dx.t + u.y * Dy.t) + u.z * Dz.t;
+		uD.x = (u.x * Dx.x + u.y * Dy.x) + u.z * Dz.x;
+		uD.y = (u.x * Dx.y + u.y * Dy.y) + u.z * Dz.y;
+		uD.z = (u.x * Dx.z + u.y * Dy.z) + u.z * Dz.z;
+
+		tmp = f * ((((-uD.t) + lx * (uD.x - hDxu)) + ly * (uD.y - hDyu)) +
+		           lz * (uD.z - hDzu));
+	}
+	V4<T> acc = {uD.t - tmp, (hDxu - uD.x) + lx * tmp,
+	             (hDyu - uD.y) + ly * tmp, (hDzu - uD.z) + lz * tmp};
+	return acc;
+}
+
+/* sim-org/phys.cl:48-64 with the selectable escape rule. */
+template <class T>
+__device__ __forceinline__ T getdt(const RayParams &p, const V4<T> &q,
+                                   const V4<T> &u, T dt)
+{
+	T r   = Mth<T>::sqrt_(getrr(T(p.a), q));
+	T eps = r - T(p.r_h);
+
+	if (eps < T(p.eps_stop))
+		return T(0.0);
+	if (p.escape == GRAYHIP_ESC_OUTWARD) {
+		T qu = (q.x * u.x + q.y * u.y) + q.z * u.z;
+		if (r > T(p.r_esc) && qu * dt > T(0.0))
+			return T(0.0);
+	} else {
+		T qq = (q.x * q.x + q.y * q.y) + q.z * q.z;
+		if (qq < T(0.0) && r > T(1e3))
+			return T(0.0);
+	}
+	T cap = (T(0.01) * eps) * eps;
+	if (fabs(dt) > cap)
+		return dt > T(0.0) ? cap : -cap;
+	return dt;
+}
+
+/* Synthetic disk: n = r^-1.5 exp(-z^2/(2 (0.3 R)^2)) outside r_isco, zero
+ * on the axis or where the Keplerian velocity is not timelike.  Keplerian
+ * Omega = 1/(r^1.5 + a); v = (1, -y Omega, x Omega, 0), u = v/sqrt(-g(v,v));
+ * shift = -k.u  (sim-org/flow.cl:36-58 with the fluid model replaced). */
+template <class T>
+__device__ __forceinline__ void flow_disk(const RayParams &p, const V4<T> &q,
+                                          const V4<T> &k, T &n, T &shift)
+{
+	typedef Mth<T> M;
+	T a  = T(p.a);
+	T r  = M::sqrt_(getrr(a, q));
+	T R2 = q.x * q.x + q.y * q.y;
+	T rs = r * M::sqrt_(r);
+
+	n = T(0.0);
+	shift = T(1.0);
+	if (!(r >= T(p.r_isco)) || !(R2 > T(0.0)))
+		return;
+	T Om = M::div(T(1.0), rs + a);
+	V4<T> v = {T(1.0), -(q.y * Om), q.x * Om, T(0.0)};
+	T vv = getuu(a, q, v);
+	if (!(vv < T(0.0)))
+		return;
+	T ut = M::div(T(1.0), M::sqrt_(-vv));
+	T sh = -(dot4(k, v) * ut);
+	if (!(sh > T(0.0)))
+		return;
+	shift = sh;
+	n = M::div(T(1.0), rs) * M::exp_(M::div(-(q.z * q.z), T(0.18) * R2));
+}
+
+/* sim-org/rt.cl:185-198 transfer structure with the synthetic emissivity
+ * j = n nu^-1/2 and absorption alpha = 0.1 n nu^-5/2 at nu = nu_i shift:
+ *   dI/dl = -j exp(-tau) / (shift^2 + 1e-28),  dtau/dl = -alpha shift.
+ * Fills dI[0..NF), dtau[0..NF). */
+template <class T, int NF>
+__device__ __forceinline__ void rt_rhs(const RayParams &p, const V4<T> &q,
+                                       const V4<T> &u, const T *tau,
+                                       T *dI, T *dtau)
+{
+	typedef Mth<T> M;
+	T n = T(0.0), shift = T(1.0);
+	if (p.fluid == GRAYHIP_FLUID_DISK) {
+		V4<T> k = down(T(p.a), q, u);
+		flow_disk(p, q, k, n, shift);
+	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
+		n = T(1.0);
+	}
+	if (n == T(0.0)) {
+#pragma unroll
+		for (int i = 0; i < NF; ++i) {
+			dI[i] = T(0.0);
+			dtau[i] = T(0.0);
+		}
+		return;
+	}
+	T den = shift * shift + T(1e-28);
+#pragma unroll
+	for (int i = 0; i < NF; ++i) {
+		T nu = T(p.nu[i]) * shift;
+		T sq = M::sqrt_(nu);
+		T j  = M::div(n, sq);
+		T al = M::div(T(0.1) * n, (nu * nu) * sq);
+		dI[i]   = M::div(-(j * M::exp_(-tau[i])), den);
+		dtau[i] = -(al * shift);
+	}
+}
+
+} // namespace gray

@@ -1,0 +1,434 @@
+/*
+ * grayhip_host.hip -- the C-ABI device layer (include/grayhip.h).
+ *
+ * Owns the device buffers of one render (SoA FP64 ray state, info, per-ray
+ * step counters, the FP32 pass's fallback queue) and one HIP stream, and
+ * sequences the kernels.  Replaces the lux-OpenCL services used by
+ * sim-org/gray.c, icond.c, evolve.c and io.c:dump.
+ */
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "launch.h"
+
+using namespace gray;
+
+struct grayhip_ctx {
+	grayhip_config cfg;
+	RayParams p;
+	size_t n_rays;
+	int nd;
+	double *d_data;
+	double *d_info;
+	int64_t *d_steps;
+	Counters *d_ctr;      /* counters of the FP32 pass                       */
+	Fallback *d_fb;
+	void *d_tmp;          /* AoS staging for read/write                      */
+	hipStream_t stream;
+	hipEvent_t ev[4];
+	bool have_icond;
+	int pool_blocks;
+	int64_t last_fallback;
+	Counters last_ctr;
+	double kern_ns;
+	int kern_launches;
+	char err[512];
+};
+
+static thread_local char g_err[512];
+
+static int fail(grayhip_ctx *c, int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(c ? c->err : g_err, 512, fmt, ap);
+	va_end(ap);
+	return code;
+}
+
+#define HCK(ctx, expr)                                                                    \
+	do {                                                                              \
+		hipError_t e_ = (expr);                                                   \
+		if (e_ != hipSuccess)                                                     \
+			return fail(ctx, GRAYHIP_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+	} while (0)
+
+/* Prograde ISCO (Bardeen, Press & Teukolsky 1972), M = 1. */
+static double r_isco(double a)
+{
+	double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+	double z2 = sqrt(3.0 * a * a + z1 * z1);
+	return 3.0 + z2 - sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
+extern "C" int grayhip_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess)
+		return 0;
+	return n;
+}
+
+extern "C" const char *grayhip_global_error(void) { return g_err; }
+
+extern "C" const char *grayhip_last_error(const grayhip_ctx *ctx)
+{
+	return ctx ? ctx->err : g_err;
+}
+
+extern "C" int grayhip_n_data(const grayhip_ctx *ctx) { return ctx ? ctx->nd : -1; }
+
+static void release(grayhip_ctx *c)
+{
+	if (!c)
+		return;
+	(void)hipFree(c->d_data);
+	(void)hipFree(c->d_info);
+	(void)hipFree(c->d_steps);
+	(void)hipFree(c->d_ctr);
+	(void)hipFree(c->d_fb);
+	(void)hipFree(c->d_tmp);
+	for (int i = 0; i < 4; ++i)
+		if (c->ev[i])
+			(void)hipEventDestroy(c->ev[i]);
+	if (c->stream)
+		(void)hipStreamDestroy(c->stream);
+	free(c);
+}
+
+extern "C" void grayhip_destroy(grayhip_ctx *ctx)
+{
+	if (ctx)
+		(void)hipSetDevice(ctx->cfg.device);
+	release(ctx);
+}
+
+extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
+{
+	if (!cfg || !out)
+		return fail(nullptr, GRAYHIP_EINVAL, "null argument");
+	*out = nullptr;
+	if (cfg->w_rays == 0 || cfg->h_rays == 0)
+		return fail(nullptr, GRAYHIP_EINVAL, "w_rays and h_rays must be positive");
+	if (cfg->w_rays > INT_MAX || cfg->h_rays > INT_MAX ||
+	    cfg->w_rays * cfg->h_rays > (size_t)INT_MAX)
+		return fail(nullptr, GRAYHIP_EINVAL, "too many rays (W*H must fit in int)");
+	if (cfg->precision != GRAYHIP_FP32 && cfg->precision != GRAYHIP_FP64)
+		return fail(nullptr, GRAYHIP_EINVAL, "precision must be 4 or 8, got %d", cfg->precision);
+	if (cfg->n_freq < 0 || cfg->n_freq > GRAYHIP_MAX_FREQ)
+		return fail(nullptr, GRAYHIP_EINVAL, "n_freq must be in [0, %d]", GRAYHIP_MAX_FREQ);
+	if (!(fabs(cfg->a_spin) < 1.0))
+		return fail(nullptr, GRAYHIP_EINVAL, "|a_spin| must be < 1");
+	if (cfg->escape != GRAYHIP_ESC_REF && cfg->escape != GRAYHIP_ESC_OUTWARD)
+		return fail(nullptr, GRAYHIP_EINVAL, "unknown escape rule %d", cfg->escape);
+	if (cfg->fluid < GRAYHIP_FLUID_NONE || cfg->fluid > GRAYHIP_FLUID_UNIFORM)
+		return fail(nullptr, GRAYHIP_EINVAL, "unknown fluid model %d", cfg->fluid);
+	if (cfg->n_freq > 0 && cfg->fluid == GRAYHIP_FLUID_NONE)
+		return fail(nullptr, GRAYHIP_EINVAL, "radiative transfer needs a fluid model");
+	for (int i = 0; i < cfg->n_freq; ++i)
+		if (!(cfg->nu[i] > 0.0))
+			return fail(nullptr, GRAYHIP_EINVAL, "frequency channel %d must be positive", i);
+
+	int ndev = grayhip_device_count();
+	if (ndev <= 0)
+		return fail(nullptr, GRAYHIP_ENODEV, "no HIP device visible");
+	if (cfg->device < 0 || cfg->device >= ndev)
+		return fail(nullptr, GRAYHIP_EINVAL, "device %d out of range (%d visible)", cfg->device, ndev);
+
+	grayhip_ctx *c = (grayhip_ctx *)calloc(1, sizeof(grayhip_ctx));
+	if (!c)
+		return fail(nullptr, GRAYHIP_ENOMEM, "host allocation failed");
+	c->cfg = *cfg;
+	c->n_rays = cfg->w_rays * cfg->h_rays;
+	c->nd = 8 + 2 * cfg->n_freq;
+
+	RayParams &p = c->p;
+	memset(&p, 0, sizeof(p));
+	p.a = cfg->a_spin;
+	p.r_h = 1.0 + sqrt(1.0 - p.a * p.a);
+	p.escape = cfg->escape;
+	p.eps_stop = cfg->eps_stop > 0.0 ? cfg->eps_stop
+	             : (cfg->escape == GRAYHIP_ESC_OUTWARD ? 0.01 * p.r_h : 0.01);
+	p.r_esc = cfg->r_esc;
+	p.r_isco = r_isco(p.a);
+	p.fluid = cfg->n_freq > 0 ? cfg->fluid : GRAYHIP_FLUID_NONE;
+	p.n_freq = cfg->n_freq;
+	for (int i = 0; i < cfg->n_freq; ++i)
+		p.nu[i] = cfg->nu[i];
+	p.w_rays = (int)cfg->w_rays;
+	p.h_rays = (int)cfg->h_rays;
+
+	int rc;
+	hipError_t e;
+	if ((e = hipSetDevice(cfg->device)) != hipSuccess) {
+		rc = fail(nullptr, GRAYHIP_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+		goto bad;
+	}
+	{
+		size_t n = c->n_rays, nd = (size_t)c->nd;
+		struct { void **ptr; size_t bytes; } al[] = {
+			{(void **)&c->d_data, n * nd * sizeof(double)},
+			{(void **)&c->d_info, n * sizeof(double)},
+			{(void **)&c->d_steps, n * sizeof(int64_t)},
+			{(void **)&c->d_ctr, sizeof(Counters)},
+			{(void **)&c->d_fb, n * sizeof(Fallback)},
+			{(void **)&c->d_tmp, n * nd * sizeof(double)},
+		};
+		for (auto &a : al) {
+			if ((e = hipMalloc(a.ptr, a.bytes)) != hipSuccess) {
+				rc = fail(nullptr, GRAYHIP_ENOMEM, "hipMalloc(%zu): %s", a.bytes,
+				          hipGetErrorString(e));
+				goto bad;
+			}
+		}
+	}
+	if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+		rc = fail(nullptr, GRAYHIP_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+		goto bad;
+	}
+	for (int i = 0; i < 4; ++i)
+		if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) {
+			rc = fail(nullptr, GRAYHIP_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
+			goto bad;
+		}
+	{
+		int bpc = 0, n_cu = 0;
+		if ((e = pool_f32_blocks_per_cu(c->p.n_freq, &bpc)) != hipSuccess ||
+		    (e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount,
+		                               cfg->device)) != hipSuccess) {
+			rc = fail(nullptr, GRAYHIP_EDEVICE, "occupancy query: %s", hipGetErrorString(e));
+			goto bad;
+		}
+		if (bpc < 1)
+			bpc = 1;
+		size_t need = (c->n_rays + 255) / 256;
+		size_t full = (size_t)bpc * (size_t)n_cu;
+		c->pool_blocks = (int)(need < full ? need : full);
+	}
+	*out = c;
+	return GRAYHIP_OK;
+bad:
+	release(c);
+	return rc;
+}
+
+extern "C" int grayhip_icond(grayhip_ctx *c, double w_img, double h_img, double r_obs,
+                             double i_obs, double j_obs)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	if (!(r_obs > 0.0))
+		return fail(c, GRAYHIP_EINVAL, "r_obs must be positive");
+	const double d2r = M_PI / 180.0;
+	c->p.w_img = w_img;
+	c->p.h_img = h_img;
+	c->p.r_obs = r_obs;
+	c->p.si = sin(d2r * i_obs);
+	c->p.ci = cos(d2r * i_obs);
+	c->p.sj = sin(d2r * j_obs);
+	c->p.cj = cos(d2r * j_obs);
+	HCK(c, hipSetDevice(c->cfg.device));
+	HCK(c, launch_icond(c->p, c->d_data, c->d_info, c->d_steps, c->n_rays, c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	c->have_icond = true;
+	return GRAYHIP_OK;
+}
+
+static int make_rule(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, StepRule *sr)
+{
+	if (!(dt != 0.0) || !std::isfinite(dt))
+		return fail(c, GRAYHIP_EINVAL, "dt must be finite and non-zero");
+	if (n_sub == 0 || n_sub > (size_t)INT_MAX)
+		return fail(c, GRAYHIP_EINVAL, "n_sub must be in [1, INT_MAX]");
+	if (n_dump == 0 || n_dump > (size_t)INT_MAX)
+		return fail(c, GRAYHIP_EINVAL, "n_dump must be in [1, INT_MAX]");
+	sr->dt = dt;
+	sr->dts = dt / (double)n_sub;
+	sr->n = (int)((INT_MAX / (long)n_sub) * (long)n_sub);
+	sr->n_dump = (int)n_dump;
+	return GRAYHIP_OK;
+}
+
+/* FP32 form: persistent pool over all rays, then FP64 re-integration of the
+ * rays it handed over. */
+static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
+{
+	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+	HCK(c, hipEventRecord(c->ev[0], c->stream));
+	HCK(c, hipEventRecord(c->ev[2], c->stream));
+	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, c->d_ctr,
+	                       c->d_fb, (unsigned)c->pool_blocks, c->stream));
+	HCK(c, hipEventRecord(c->ev[3], c->stream));
+	Counters h;
+	HCK(c, hipMemcpyAsync(&h, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	const unsigned nfb = h.n_fb;
+	if (nfb > c->n_rays)
+		return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow (%u)", nfb);
+	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, c->d_fb,
+	                           c->d_ctr, nfb, c->stream));
+	HCK(c, hipEventRecord(c->ev[1], c->stream));
+	HCK(c, hipMemcpyAsync(&c->last_ctr, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost,
+	                      c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	float ms = 0.f, kms = 0.f;
+	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+	HCK(c, hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
+	c->kern_ns = 1e6 * (double)kms;
+	c->kern_launches = 1;
+	c->last_fallback = nfb;
+	if (ns)
+		*ns = 1e6 * (double)ms;
+	return GRAYHIP_OK;
+}
+
+/* Plain FP64 form: one launch per interval. */
+static int run_fp64(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
+{
+	HCK(c, hipEventRecord(c->ev[0], c->stream));
+	for (size_t d = 0; d < n_dump; ++d)
+		HCK(c, launch_evolve_f64(c->p, one, c->d_data, c->d_info, c->d_steps, c->n_rays,
+		                         c->stream));
+	HCK(c, hipEventRecord(c->ev[1], c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	float ms = 0.f;
+	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+	c->kern_ns = 1e6 * (double)ms;
+	c->kern_launches = (int)n_dump;
+	c->last_fallback = 0;
+	memset(&c->last_ctr, 0, sizeof(c->last_ctr));
+	if (ns)
+		*ns = 1e6 * (double)ms;
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_evolve(grayhip_ctx *c, double dt, size_t n_sub, double *ns)
+{
+	return grayhip_render(c, dt, n_sub, 1, ns);
+}
+
+extern "C" int grayhip_render(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, double *ns)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	if (!c->have_icond)
+		return fail(c, GRAYHIP_ESTATE, "grayhip_icond (or grayhip_write) must come first");
+	StepRule sr;
+	int rc = make_rule(c, dt, n_sub, n_dump, &sr);
+	if (rc)
+		return rc;
+	HCK(c, hipSetDevice(c->cfg.device));
+	if (c->cfg.precision == GRAYHIP_FP64) {
+		sr.n_dump = 1;
+		return run_fp64(c, sr, n_dump, ns);
+	}
+	return run_fp32(c, sr, ns);
+}
+
+extern "C" int grayhip_read(grayhip_ctx *c, void *data, void *info)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	const int sz = c->cfg.precision;
+	HCK(c, hipSetDevice(c->cfg.device));
+	if (data) {
+		HCK(c, launch_soa_to_aos(c->d_data, c->d_tmp, sz, c->n_rays, c->nd, c->stream));
+		HCK(c, hipMemcpyAsync(data, c->d_tmp, c->n_rays * (size_t)c->nd * sz,
+		                      hipMemcpyDeviceToHost, c->stream));
+		HCK(c, hipStreamSynchronize(c->stream));
+	}
+	if (info) {
+		HCK(c, launch_cast(c->d_info, c->d_tmp, sz, c->n_rays, c->stream));
+		HCK(c, hipMemcpyAsync(info, c->d_tmp, c->n_rays * (size_t)sz, hipMemcpyDeviceToHost,
+		                      c->stream));
+		HCK(c, hipStreamSynchronize(c->stream));
+	}
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_read_f64(grayhip_ctx *c, double *data, double *info)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	HCK(c, hipSetDevice(c->cfg.device));
+	if (data) {
+		HCK(c, launch_soa_to_aos(c->d_data, c->d_tmp, 8, c->n_rays, c->nd, c->stream));
+		HCK(c, hipMemcpyAsync(data, c->d_tmp, c->n_rays * (size_t)c->nd * 8,
+		                      hipMemcpyDeviceToHost, c->stream));
+	}
+	if (info)
+		HCK(c, hipMemcpyAsync(info, c->d_info, c->n_rays * 8, hipMemcpyDeviceToHost, c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_write(grayhip_ctx *c, const void *data)
+{
+	if (!c || !data)
+		return fail(c, GRAYHIP_EINVAL, "null argument");
+	const int sz = c->cfg.precision;
+	HCK(c, hipSetDevice(c->cfg.device));
+	HCK(c, hipMemcpyAsync(c->d_tmp, data, c->n_rays * (size_t)c->nd * sz, hipMemcpyHostToDevice,
+	                      c->stream));
+	HCK(c, launch_aos_to_soa(c->d_tmp, c->d_data, sz, c->n_rays, c->nd, c->stream));
+	HCK(c, launch_info_f64(c->p, c->d_data, c->d_info, c->n_rays, c->stream));
+	HCK(c, hipMemsetAsync(c->d_steps, 0, c->n_rays * sizeof(int64_t), c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	c->have_icond = true;
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_read_steps(grayhip_ctx *c, int64_t *steps)
+{
+	if (!c || !steps)
+		return fail(c, GRAYHIP_EINVAL, "null argument");
+	HCK(c, hipSetDevice(c->cfg.device));
+	HCK(c, hipMemcpy(steps, c->d_steps, c->n_rays * sizeof(int64_t), hipMemcpyDeviceToHost));
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_stats(grayhip_ctx *c, int64_t *steps_total, int64_t *fallback_rays)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	if (steps_total) {
+		std::vector<int64_t> s(c->n_rays);
+		int rc = grayhip_read_steps(c, s.data());
+		if (rc)
+			return rc;
+		int64_t t = 0;
+		for (int64_t v : s)
+			t += v;
+		*steps_total = t;
+	}
+	if (fallback_rays)
+		*fallback_rays = c->last_fallback;
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_pass_steps(grayhip_ctx *c, int64_t *pool_steps, int64_t *fallback_steps)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	if (pool_steps)
+		*pool_steps = (int64_t)c->last_ctr.pool_steps;
+	if (fallback_steps)
+		*fallback_steps = (int64_t)c->last_ctr.fb_steps;
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_kernel_time(grayhip_ctx *c, double *ns, int *launches)
+{
+	if (!c)
+		return fail(nullptr, GRAYHIP_EINVAL, "null context");
+	if (ns)
+		*ns = c->kern_ns;
+	if (launches)
+		*launches = c->kern_launches;
+	return GRAYHIP_OK;
+}

@@ -1,0 +1,226 @@
+/*
+ * kernels_f32.hip -- the FP32 form: a persistent ray-pool kernel.
+ *
+ * Each lane owns one ray at a time.  A wave refills its idle lanes together:
+ * one ballot collects the lanes whose ray retired, one atomicAdd on the
+ * global ray counter reserves that many ray indices, and each lane takes
+ * base + (its rank among the idle lanes).  Waves exit when the counter has
+ * passed the last ray and all their lanes are idle, so every wave ends.
+ *
+ * Arithmetic: the RK4 stages and the geodesic / transfer RHS run in FP32
+ * with the hardware rcp/sqrt/exp; the state is carried as an unevaluated
+ * sum hi + lo of two floats (Kahan-compensated update), which keeps the
+ * position at r ~ 1000 and the slowly varying momentum accurate far below
+ * one FP32 ulp per step.  Camera initial data come from the FP64 icond
+ * kernel; the quantised step schedule (sim-org/driver.cl:81-99) is computed
+ * in FP64.  A ray whose step falls below 1e-4 M (horizon approach) is handed
+ * to the FP64 kernel, which continues it from its compensated state
+ * (hi + lo promoted to double); a ray whose FP32 state turns non-finite is
+ * redone in FP64 from the state the pass started from.
+ */
+#include <climits>
+
+#include "gray_device.h"
+#include "launch.h"
+
+namespace gray {
+
+template <int NF>
+__device__ __forceinline__ void rhs_f32(const RayParams &p, const float *s, float *ds)
+{
+	V4<float> q = {s[0], s[1], s[2], s[3]};
+	V4<float> u = {s[4], s[5], s[6], s[7]};
+	V4<float> a = gr_rhs((float)p.a, q, u);
+	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
+	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
+	if (NF > 0)
+		rt_rhs<float, NF>(p, q, u, s + 8 + NF, ds + 8, ds + 8 + NF);
+}
+
+/* Classical RK4 (sim-org/RK4.cl:37-47) with stages taken from hi and the
+ * increment added to (hi, lo) by compensated summation. */
+template <int NF>
+__device__ __forceinline__ void rk4_f32c(const RayParams &p, float *hi, float *lo, float dt)
+{
+	constexpr int ND = 8 + 2 * NF;
+	float k[ND], acc[ND], x[ND];
+	const float hdt = 0.5f * dt;
+
+	rhs_f32<NF>(p, hi, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] = k[i];
+		x[i] = hi[i] + hdt * k[i];
+	}
+	rhs_f32<NF>(p, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] += 2.0f * k[i];
+		x[i] = hi[i] + hdt * k[i];
+	}
+	rhs_f32<NF>(p, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] += 2.0f * k[i];
+		x[i] = hi[i] + dt * k[i];
+	}
+	rhs_f32<NF>(p, x, k);
+	const float dt6 = dt * (1.0f / 6.0f);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		float y = (acc[i] + k[i]) * dt6 + lo[i];
+		float t = hi[i] + y;
+		lo[i] = y - (t - hi[i]);
+		hi[i] = t;
+	}
+}
+
+template <int NF>
+__device__ __forceinline__ void store_state(double *__restrict__ data, size_t n_rays, size_t g,
+                                            const float *hi, const float *lo)
+{
+	constexpr int ND = 8 + 2 * NF;
+#pragma unroll
+	for (int i = 0; i < ND; ++i)
+		data[(size_t)i * n_rays + g] = (double)hi[i] + (double)lo[i];
+}
+
+template <int NF>
+__global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
+                                                  double *__restrict__ data,
+                                                  double *__restrict__ info,
+                                                  int64_t *__restrict__ steps,
+                                                  size_t n_rays, Counters *__restrict__ ctr,
+                                                  Fallback *__restrict__ fb)
+{
+	constexpr int ND = 8 + 2 * NF;
+	const unsigned lane = __lane_id();
+	const float dts = (float)sr.dts;
+	const double dt_over_n = sr.dt / (double)sr.n;
+
+	float hi[ND], lo[ND];
+	long g = -1;
+	int d = 0, h = 0;
+	int64_t st = 0, st0 = 0;
+	bool exhausted = false;
+
+	for (;;) {
+		/* wave-level refill of retired lanes */
+		const bool need = (g < 0) && !exhausted;
+		const unsigned long long m = __ballot(need);
+		if (m) {
+			const int leader = __ffsll((unsigned long long)m) - 1;
+			unsigned base = 0;
+			if ((int)lane == leader)
+				base = atomicAdd(&ctr->ray_next, (unsigned)__popcll(m));
+			base = __shfl(base, leader);
+			if (need) {
+				unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+				if ((size_t)idx < n_rays) {
+					g = idx;
+#pragma unroll
+					for (int i = 0; i < ND; ++i) {
+						double v = data[(size_t)i * n_rays + g];
+						hi[i] = (float)v;
+						lo[i] = (float)(v - (double)hi[i]);
+					}
+					st = steps[g];
+					st0 = st;
+					d = 0;
+					h = 0;
+				} else {
+					exhausted = true;
+				}
+			}
+		}
+		if (g < 0)
+			break;
+
+		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
+		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
+		float gdt = getdt(p, q, u, dts);
+		double xd = (double)gdt / sr.dt * (double)sr.n;
+		int dh = (xd == xd) ? (int)xd : 0;
+		bool finish = (dh == 0);
+		if (!finish) {
+			if (dh > sr.n - h)
+				dh = sr.n - h;
+			double hd = (double)dh * dt_over_n;
+			if (fabs(hd) < 1e-4) {
+				/* step collapse: FP64 re-integration from here */
+				store_state<NF>(data, n_rays, g, hi, lo);
+				steps[g] = st;
+				atomicAdd(&ctr->pool_steps, (unsigned long long)(st - st0));
+				unsigned slot = atomicAdd(&ctr->n_fb, 1u);
+				fb[slot] = Fallback{(int)g, d, h, 0};
+				g = -1;
+				continue;
+			}
+			rk4_f32c<NF>(p, hi, lo, (float)hd);
+			++st;
+			float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
+			if (!__builtin_isfinite(chk)) {
+				/* the stored state is still the one this pass started
+				 * from: redo the whole pass in FP64 */
+				unsigned slot = atomicAdd(&ctr->n_fb, 1u);
+				fb[slot] = Fallback{(int)g, 0, 0, 0};
+				g = -1;
+				continue;
+			}
+			h += dh;
+			if (h >= sr.n) {
+				h = 0;
+				finish = (++d >= sr.n_dump);
+			}
+		}
+		if (finish) {
+			store_state<NF>(data, n_rays, g, hi, lo);
+			V4<double> qd = {(double)hi[0] + (double)lo[0], (double)hi[1] + (double)lo[1],
+			                 (double)hi[2] + (double)lo[2], (double)hi[3] + (double)lo[3]};
+			V4<double> ud = {(double)hi[4] + (double)lo[4], (double)hi[5] + (double)lo[5],
+			                 (double)hi[6] + (double)lo[6], (double)hi[7] + (double)lo[7]};
+			info[g] = getuu(p.a, qd, ud);
+			steps[g] = st;
+			atomicAdd(&ctr->pool_steps, (unsigned long long)(st - st0));
+			g = -1;
+		}
+	}
+}
+
+#define GRAY_NF_SWITCH(NFV, CALL)                       \
+	switch (NFV) {                                  \
+	case 0: CALL(0); break;                         \
+	case 1: CALL(1); break;                         \
+	case 2: CALL(2); break;                         \
+	case 3: CALL(3); break;                         \
+	case 4: CALL(4); break;                         \
+	case 5: CALL(5); break;                         \
+	case 6: CALL(6); break;                         \
+	case 7: CALL(7); break;                         \
+	case 8: CALL(8); break;                         \
+	default: return hipErrorInvalidValue;           \
+	}
+
+hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks)
+{
+	*blocks = 0;
+#define L(NF) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_pool_f32<NF>, 256, 0)
+	GRAY_NF_SWITCH(n_freq, L)
+#undef L
+	return hipSuccess;
+}
+
+hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
+                           int64_t *steps, size_t n_rays, Counters *ctr, Fallback *fb,
+                           unsigned n_blocks, hipStream_t st)
+{
+	if (n_rays == 0 || n_blocks == 0)
+		return hipSuccess;
+#define L(NF) hipLaunchKernelGGL(k_pool_f32<NF>, dim3(n_blocks), dim3(256), 0, st, \
+	                         p, sr, data, info, steps, n_rays, ctr, fb)
+	GRAY_NF_SWITCH(p.n_freq, L)
+#undef L
+	return hipGetLastError();
+}
+
+} // namespace gray

@@ -1,0 +1,33 @@
+/*
+ * launch.h -- host-side launchers of the gfx950 kernels (one per kernel
+ * family), called by the C-ABI layer.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "grayhip_params.h"
+
+namespace gray {
+
+hipError_t launch_icond(const RayParams &p, double *data, double *info, int64_t *steps,
+                        size_t n_rays, hipStream_t st);
+hipError_t launch_evolve_f64(const RayParams &p, const StepRule &sr, double *data, double *info,
+                             int64_t *steps, size_t n_rays, hipStream_t st);
+hipError_t launch_fallback_f64(const RayParams &p, const StepRule &sr, double *data, double *info,
+                               int64_t *steps, size_t n_rays, const Fallback *fb,
+                               Counters *ctr, unsigned max_fb, hipStream_t st);
+hipError_t launch_soa_to_aos(const double *soa, void *aos, int real_size, size_t n_rays, int nd,
+                             hipStream_t st);
+hipError_t launch_aos_to_soa(const void *aos, double *soa, int real_size, size_t n_rays, int nd,
+                             hipStream_t st);
+hipError_t launch_cast(const double *in, void *out, int real_size, size_t n, hipStream_t st);
+hipError_t launch_info_f64(const RayParams &p, const double *data, double *info, size_t n_rays,
+                           hipStream_t st);
+
+hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
+hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
+                           int64_t *steps, size_t n_rays, Counters *ctr, Fallback *fb,
+                           unsigned n_blocks, hipStream_t st);
+
+} // namespace gray

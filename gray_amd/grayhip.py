@@ -1,0 +1,191 @@
+"""ctypes binding of the gfx950 device layer libgrayhip.so (include/grayhip.h).
+
+The library is built in-tree by ``make -C gray_amd``; importing this module
+without it raises, so nothing silently falls back to CPU code.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgrayhip.so")
+
+MAX_FREQ = 8
+FP32, FP64 = 4, 8
+ESC_REF, ESC_OUTWARD = 0, 1
+FLUID_NONE, FLUID_DISK, FLUID_UNIFORM = 0, 1, 2
+
+# Every entry point declared in include/grayhip.h.
+EXPORTS = (
+    "grayhip_create", "grayhip_destroy", "grayhip_icond", "grayhip_evolve",
+    "grayhip_render", "grayhip_read", "grayhip_read_f64", "grayhip_write",
+    "grayhip_read_steps", "grayhip_stats", "grayhip_pass_steps", "grayhip_kernel_time",
+    "grayhip_n_data",
+    "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
+)
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("w_rays", C.c_size_t),
+        ("h_rays", C.c_size_t),
+        ("precision", C.c_int),
+        ("a_spin", C.c_double),
+        ("n_freq", C.c_int),
+        ("nu", C.c_double * MAX_FREQ),
+        ("fluid", C.c_int),
+        ("escape", C.c_int),
+        ("r_esc", C.c_double),
+        ("eps_stop", C.c_double),
+        ("device", C.c_int),
+    ]
+
+
+class GrayHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libgrayhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GrayHipError(f"{LIB_PATH} is missing: build it with `make -C gray_amd`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    L.grayhip_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.grayhip_destroy.argtypes = [vp]
+    L.grayhip_destroy.restype = None
+    L.grayhip_icond.argtypes = [vp] + [C.c_double] * 5
+    L.grayhip_evolve.argtypes = [vp, C.c_double, C.c_size_t, dp]
+    L.grayhip_render.argtypes = [vp, C.c_double, C.c_size_t, C.c_size_t, dp]
+    L.grayhip_read.argtypes = [vp, vp, vp]
+    L.grayhip_read_f64.argtypes = [vp, vp, vp]
+    L.grayhip_write.argtypes = [vp, vp]
+    L.grayhip_read_steps.argtypes = [vp, vp]
+    L.grayhip_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.grayhip_pass_steps.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.grayhip_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int)]
+    L.grayhip_n_data.argtypes = [vp]
+    L.grayhip_last_error.argtypes = [vp]
+    L.grayhip_last_error.restype = C.c_char_p
+    L.grayhip_global_error.argtypes = []
+    L.grayhip_global_error.restype = C.c_char_p
+    L.grayhip_device_count.argtypes = []
+    _lib = L
+    return L
+
+
+def device_count():
+    return lib().grayhip_device_count()
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Render:
+    """One render on the GPU: W*H rays of one camera, one spacetime."""
+
+    def __init__(self, w_rays, h_rays, a_spin, precision=FP64, nu=(), fluid=None,
+                 escape=ESC_OUTWARD, r_esc=1000.0, eps_stop=0.0, device=0):
+        cfg = Config()
+        cfg.w_rays, cfg.h_rays = w_rays, h_rays
+        cfg.precision = precision
+        cfg.a_spin = a_spin
+        cfg.n_freq = len(nu)
+        for i, v in enumerate(nu):
+            cfg.nu[i] = v
+        if fluid is None:
+            fluid = FLUID_DISK if len(nu) else FLUID_NONE
+        cfg.fluid = fluid
+        cfg.escape = escape
+        cfg.r_esc = r_esc
+        cfg.eps_stop = eps_stop
+        cfg.device = device
+        self.cfg = cfg
+        self.n_rays = w_rays * h_rays
+        self.n_data = 8 + 2 * len(nu)
+        self._h = C.c_void_p()
+        rc = lib().grayhip_create(C.byref(cfg), C.byref(self._h))
+        if rc != 0:
+            raise GrayHipError(f"grayhip_create: {rc}: {lib().grayhip_global_error().decode()}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GrayHipError(f"{what}: {rc}: {lib().grayhip_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            lib().grayhip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def icond(self, w_img, h_img, r_obs, i_obs, j_obs):
+        self._check(lib().grayhip_icond(self._h, w_img, h_img, r_obs, i_obs, j_obs), "icond")
+
+    def evolve(self, dt, n_sub):
+        ns = C.c_double()
+        self._check(lib().grayhip_evolve(self._h, dt, n_sub, C.byref(ns)), "evolve")
+        return ns.value
+
+    def render(self, dt, n_sub, n_dump):
+        ns = C.c_double()
+        self._check(lib().grayhip_render(self._h, dt, n_sub, n_dump, C.byref(ns)), "render")
+        return ns.value
+
+    def read(self):
+        """(data AoS [n_rays, n_data], info [n_rays]) in the configured precision."""
+        t = np.float64 if self.cfg.precision == FP64 else np.float32
+        data = np.empty((self.n_rays, self.n_data), dtype=t)
+        info = np.empty(self.n_rays, dtype=t)
+        self._check(lib().grayhip_read(self._h, _ptr(data), _ptr(info)), "read")
+        return data, info
+
+    def read_f64(self):
+        data = np.empty((self.n_rays, self.n_data), dtype=np.float64)
+        info = np.empty(self.n_rays, dtype=np.float64)
+        self._check(lib().grayhip_read_f64(self._h, _ptr(data), _ptr(info)), "read_f64")
+        return data, info
+
+    def write(self, data):
+        t = np.float64 if self.cfg.precision == FP64 else np.float32
+        data = np.ascontiguousarray(data, dtype=t)
+        assert data.shape == (self.n_rays, self.n_data)
+        self._check(lib().grayhip_write(self._h, _ptr(data)), "write")
+
+    def steps(self):
+        s = np.empty(self.n_rays, dtype=np.int64)
+        self._check(lib().grayhip_read_steps(self._h, _ptr(s)), "read_steps")
+        return s
+
+    def stats(self):
+        st, fb = C.c_int64(), C.c_int64()
+        self._check(lib().grayhip_stats(self._h, C.byref(st), C.byref(fb)), "stats")
+        return st.value, fb.value
+
+    def pass_steps(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._check(lib().grayhip_pass_steps(self._h, C.byref(a), C.byref(b)), "pass_steps")
+        return a.value, b.value
+
+    def kernel_time(self):
+        ns, n = C.c_double(), C.c_int()
+        self._check(lib().grayhip_kernel_time(self._h, C.byref(ns), C.byref(n)), "kernel_time")
+        return ns.value, n.value

@@ -1,0 +1,141 @@
+/*
+ * grayhip.h -- C ABI of the MI355X (gfx950) GRay2 device layer.
+ *
+ * This is the drop-in boundary below the job module: every entry point
+ * replaces one device-side service that the reference's host code obtains
+ * from lux-OpenCL.  Plain pointers and sizes only; all functions return 0 on
+ * success and a negative GRAYHIP_E* code on failure, with a message kept per
+ * context (grayhip_last_error) or per thread (grayhip_global_error).
+ *
+ * Ray order and pixel mapping are those of the reference: ray g = gi + gj*W
+ * for column gi < W and row gj < H (sim-org/driver.cl:47-57).  The state of a
+ * ray is n_data = 8 + 2*n_freq reals  t x y z u^t u^x u^y u^z I[n_freq]
+ * tau[n_freq]  (sim-org/phys.cl:30-33, rt.cl:174-177); info holds g(u,u).
+ */
+#ifndef GRAYHIP_H
+#define GRAYHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAYHIP_MAX_FREQ 8
+
+enum {
+	GRAYHIP_OK        = 0,
+	GRAYHIP_EINVAL    = -1,  /* bad argument or configuration          */
+	GRAYHIP_ENOMEM    = -2,  /* device allocation failed               */
+	GRAYHIP_EDEVICE   = -3,  /* HIP runtime error                      */
+	GRAYHIP_ESTATE    = -4,  /* call out of order (e.g. evolve before icond) */
+	GRAYHIP_ENODEV    = -5   /* no usable GPU                          */
+};
+
+/* Arithmetic forms (the `precision` option, sim-org/setup.opts:6). */
+enum {
+	GRAYHIP_FP32 = 4,  /* FP32 bulk RK4/RHS, compensated state, FP64 ICs,
+	                      FP64 re-integration of collapsed-step rays      */
+	GRAYHIP_FP64 = 8   /* plain FP64 form, bitwise equal to the oracle    */
+};
+
+/* Termination (sim-org/phys.cl:48-64). */
+enum {
+	GRAYHIP_ESC_REF     = 0,  /* as written: eps < eps_stop only (the escape
+	                             test of phys.cl:57 can never fire)       */
+	GRAYHIP_ESC_OUTWARD = 1   /* plus r > r_esc while moving outward along
+	                             the integration direction                */
+};
+
+/* Fluid feeding radiative transfer (sim-org/flow.cl:36-58 replaced). */
+enum {
+	GRAYHIP_FLUID_NONE    = 0,
+	GRAYHIP_FLUID_DISK    = 1,  /* synthetic Keplerian disk             */
+	GRAYHIP_FLUID_UNIFORM = 2   /* n = 1, shift = 1 (slab closed forms) */
+};
+
+typedef struct {
+	size_t w_rays, h_rays;   /* W, H (sim-org/param.opts:10-11)            */
+	int    precision;        /* GRAYHIP_FP64 or GRAYHIP_FP32               */
+	double a_spin;           /* sim/Kerr.opts:2                            */
+	int    n_freq;           /* 0 turns radiative transfer off             */
+	double nu[GRAYHIP_MAX_FREQ];
+	int    fluid;            /* GRAYHIP_FLUID_*                            */
+	int    escape;           /* GRAYHIP_ESC_*                              */
+	double r_esc;            /* escape radius for GRAYHIP_ESC_OUTWARD      */
+	double eps_stop;         /* horizon stop r - r_H < eps_stop; <= 0 picks
+	                            0.01 r_H for ESC_OUTWARD, 0.01 for ESC_REF  */
+	int    device;           /* HIP device ordinal                          */
+} grayhip_config;
+
+typedef struct grayhip_ctx grayhip_ctx;
+
+/* Allocate device state for W*H rays (replaces the buffer set-up of
+ * sim-org/gray.c:130-133 and the kernel specialisation of build.c:30-143:
+ * resolution, spin and channels are run-time arguments here). */
+int  grayhip_create(const grayhip_config *cfg, grayhip_ctx **out);
+
+/* Free everything (sim-org/gray.c:330-346). */
+void grayhip_destroy(grayhip_ctx *ctx);
+
+/* Camera initial conditions in FP64 (replaces sim-org/icond.c:29-86 and
+ * the icond_drv kernel, driver.cl:36-67).  Angles in degrees. */
+int  grayhip_icond(grayhip_ctx *ctx, double w_img, double h_img,
+                   double r_obs, double i_obs, double j_obs);
+
+/* One dump interval: every ray advances by affine parameter dt in the
+ * integer-quantised sub-steps of n_sub nominal steps (replaces
+ * sim-org/evolve.c:28-67 and evolve_drv, driver.cl:70-108).  *ns (may be
+ * NULL) receives the device time in nanoseconds, as the reference's
+ * evolve() returns it. */
+int  grayhip_evolve(grayhip_ctx *ctx, double dt, size_t n_sub, double *ns);
+
+/* The whole job after icond: n_dump intervals of dt each, integrated in one
+ * persistent ray-pool pass (rays retire and lanes refill independently).
+ * Equivalent to n_dump calls of grayhip_evolve without the intermediate
+ * dumps.  *ns receives the device time. */
+int  grayhip_render(grayhip_ctx *ctx, double dt, size_t n_sub, size_t n_dump,
+                    double *ns);
+
+/* Copy the ray states to the host in the reference's dump order
+ * (AoS [H][W][n_data], sim-org/io.c:61-66) as `precision`-byte reals;
+ * info (may be NULL) receives n_rays reals of g(u,u)
+ * (replaces the ocl->mmap readback of io.c:52-53). */
+int  grayhip_read(grayhip_ctx *ctx, void *data, void *info);
+
+/* Same as grayhip_read but always in FP64 (the device keeps FP64 state in
+ * both forms): data AoS [H][W][n_data] doubles, info n_rays doubles. */
+int  grayhip_read_f64(grayhip_ctx *ctx, double *data, double *info);
+
+/* Upload AoS states (same layout as grayhip_read) to restart from a dump. */
+int  grayhip_write(grayhip_ctx *ctx, const void *data);
+
+/* Per-ray count of RK4 steps executed since icond (int64[n_rays]). */
+int  grayhip_read_steps(grayhip_ctx *ctx, int64_t *steps);
+
+/* Counters of the last render/evolve: total RK4 steps, and the number of
+ * rays re-integrated in FP64 (FP32 form only). */
+int  grayhip_stats(grayhip_ctx *ctx, int64_t *steps_total,
+                   int64_t *fallback_rays);
+
+/* RK4 steps of the last FP32-form call split by where they ran: the FP32
+ * pool pass and the FP64 re-integration (both 0 after an FP64-form call). */
+int  grayhip_pass_steps(grayhip_ctx *ctx, int64_t *pool_steps,
+                        int64_t *fallback_steps);
+
+/* Device time (ns) of the dominant kernel in the last call, measured with
+ * HIP events on the stream it ran on; *launches receives its launch count. */
+int  grayhip_kernel_time(grayhip_ctx *ctx, double *ns, int *launches);
+
+int  grayhip_n_data(const grayhip_ctx *ctx);
+const char *grayhip_last_error(const grayhip_ctx *ctx);
+const char *grayhip_global_error(void);
+
+/* Number of visible HIP devices (0 when none); never aborts. */
+int  grayhip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
