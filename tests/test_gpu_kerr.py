@@ -102,9 +102,11 @@ def test_fp32_form_image_error():
     # captured rays went through the FP64 fallback
     r0 = np.sqrt([O.getrr(A, q) for q in d0[:, :4]])
     assert nfb >= (r0 < 2.0).sum() > 0
-    # final states of escaping rays agree closely
+    # final positions of escaping rays (r ~ 1000) agree to ~1e-7 relative
+    # typically and 1e-4 relative for the worst (lensed, long) ray
     esc = r0 > 999.0
-    assert np.abs(d1[esc, 1:4] - d0[esc, 1:4]).max() < 1e-3
+    dx = np.abs(d1[esc, 1:4] - d0[esc, 1:4]).max(axis=1) / r0[esc]
+    assert np.median(dx) < 1e-6 and dx.max() < 1e-4, (np.median(dx), dx.max())
     # step counts within a few per mille
     assert abs(s1.sum() / s0.sum() - 1) < 5e-3
 
