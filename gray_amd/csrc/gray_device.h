@@ -248,21 +248,24 @@ __device__ __forceinline__ T getdt(const RayParams &p, const V4<T> &q,
 /* Synthetic disk: n = r^-1.5 exp(-z^2/(2 (0.3 R)^2)) outside r_isco, zero
  * on the axis or where the Keplerian velocity is not timelike.  Keplerian
  * Omega = 1/(r^1.5 + a); v = (1, -y Omega, x Omega, 0), u = v/sqrt(-g(v,v));
- * shift = -k.u  (sim-org/flow.cl:36-58 with the fluid model replaced). */
+ * shift = -k.u with k = down(q, u_photon)  (sim-org/flow.cl:36-58 with the
+ * fluid model replaced).  Inside r_isco nothing else is evaluated, so the
+ * horizon approach pays for the geodesic only. */
 template <class T>
 __device__ __forceinline__ void flow_disk(const RayParams &p, const V4<T> &q,
-                                          const V4<T> &k, T &n, T &shift)
+                                          const V4<T> &u, T &n, T &shift)
 {
 	typedef Mth<T> M;
 	T a  = T(p.a);
 	T r  = M::sqrt_(getrr(a, q));
 	T R2 = q.x * q.x + q.y * q.y;
-	T rs = r * M::sqrt_(r);
 
 	n = T(0.0);
 	shift = T(1.0);
 	if (!(r >= T(p.r_isco)) || !(R2 > T(0.0)))
 		return;
+	V4<T> k = down(a, q, u);
+	T rs = r * M::sqrt_(r);
 	T Om = M::div(T(1.0), rs + a);
 	V4<T> v = {T(1.0), -(q.y * Om), q.x * Om, T(0.0)};
 	T vv = getuu(a, q, v);
@@ -288,8 +291,7 @@ __device__ __forceinline__ void rt_rhs(const RayParams &p, const V4<T> &q,
 	typedef Mth<T> M;
 	T n = T(0.0), shift = T(1.0);
 	if (p.fluid == GRAYHIP_FLUID_DISK) {
-		V4<T> k = down(T(p.a), q, u);
-		flow_disk(p, q, k, n, shift);
+		flow_disk(p, q, u, n, shift);
 	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
 		n = T(1.0);
 	}

@@ -1,0 +1,83 @@
+/*
+ * kernels_fb.hip -- FP64 re-integration of the rays the FP32 pass hands
+ * over (step collapse near the horizon, or a non-finite FP32 state).
+ *
+ * Same FP64 step code as the plain form (step_f64.h) but compiled with
+ * multiply-add contraction: these rays are not part of the bit-exact form.
+ */
+#include "launch.h"
+#include "step_f64.h"
+
+namespace gray {
+
+/* FP64 re-integration of rays handed over by the FP32 pass: continue from
+ * the stored state at unit h of interval `dump` through the remaining
+ * intervals. */
+template <int NF>
+__global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
+                                                      double *__restrict__ data,
+                                                      double *__restrict__ info,
+                                                      int64_t *__restrict__ steps,
+                                                      size_t n_rays,
+                                                      const Fallback *__restrict__ fb,
+                                                      Counters *__restrict__ ctr)
+{
+	constexpr int ND = 8 + 2 * NF;
+	unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= ctr->n_fb)
+		return;
+	Fallback f = fb[e];
+	size_t g = (size_t)f.g;
+	double s[ND];
+	const int d0 = f.dump, h0 = f.h;
+#pragma unroll
+	for (int i = 0; i < ND; ++i)
+		s[i] = data[(size_t)i * n_rays + g];
+	const int64_t st0 = steps[g];
+	int64_t st = st0;
+	for (int d = d0; d < sr.n_dump; ++d) {
+		bool retired;
+		st += interval_f64<NF>(p, sr, s, d == d0 ? h0 : 0, &retired);
+		if (retired)
+			break;
+	}
+#pragma unroll
+	for (int i = 0; i < ND; ++i)
+		data[(size_t)i * n_rays + g] = s[i];
+	V4<double> q = {s[0], s[1], s[2], s[3]};
+	V4<double> u = {s[4], s[5], s[6], s[7]};
+	info[g] = getuu(p.a, q, u);
+	steps[g] = st;
+	atomicAdd(&ctr->fb_steps, (unsigned long long)(st - st0));
+}
+
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+#define GRAY_NF_SWITCH(NFV, CALL)                       \
+	switch (NFV) {                                  \
+	case 0: CALL(0); break;                         \
+	case 1: CALL(1); break;                         \
+	case 2: CALL(2); break;                         \
+	case 3: CALL(3); break;                         \
+	case 4: CALL(4); break;                         \
+	case 5: CALL(5); break;                         \
+	case 6: CALL(6); break;                         \
+	case 7: CALL(7); break;                         \
+	case 8: CALL(8); break;                         \
+	default: return hipErrorInvalidValue;           \
+	}
+
+hipError_t launch_fallback_f64(const RayParams &p, const StepRule &sr, double *data, double *info,
+                               int64_t *steps, size_t n_rays, const Fallback *fb,
+                               Counters *ctr, unsigned max_fb, hipStream_t st)
+{
+	if (max_fb == 0)
+		return hipSuccess;
+#define L(NF) hipLaunchKernelGGL(k_fallback_f64<NF>, dim3(blocks_for(max_fb, 256)), dim3(256), 0, st, \
+	                         p, sr, data, info, steps, n_rays, fb, ctr)
+	GRAY_NF_SWITCH(p.n_freq, L)
+#undef L
+	return hipGetLastError();
+}
+
+} // namespace gray

@@ -1,0 +1,93 @@
+/*
+ * step_f64.h -- one ray in FP64: state derivative, classical RK4 step, the
+ * quantised sub-steps of one dump interval, and the camera initial state.
+ * Included by the bit-exact FP64 kernels (compiled without contraction)
+ * and by the FP64 re-integration of the FP32 form (compiled with it).
+ */
+#pragma once
+
+#include "gray_device.h"
+
+namespace gray {
+
+template <int NF>
+__device__ __forceinline__ void rhs_f64(const RayParams &p, const double *s, double *ds)
+{
+	V4<double> q = {s[0], s[1], s[2], s[3]};
+	V4<double> u = {s[4], s[5], s[6], s[7]};
+	V4<double> a = gr_rhs(p.a, q, u);
+	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
+	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
+	if (NF > 0)
+		rt_rhs<double, NF>(p, q, u, s + 8 + NF, ds + 8, ds + 8 + NF);
+}
+
+/* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at
+ * s + (dt/2) k1, s + (dt/2) k2, s + dt k3. */
+template <int NF>
+__device__ __forceinline__ void rk4_f64(const RayParams &p, double *s, double dt)
+{
+	constexpr int ND = 8 + 2 * NF;
+	double k1[ND], k[ND], x[ND], k23[ND];
+	const double hdt = 0.5 * dt;
+
+	rhs_f64<NF>(p, s, k1);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
+	rhs_f64<NF>(p, x, k23);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
+	rhs_f64<NF>(p, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		k23[i] = k23[i] + k[i];
+		x[i] = s[i] + dt * k[i];
+	}
+	rhs_f64<NF>(p, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i)
+		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k23[i]) + k[i])) / 6.0;
+}
+
+/* Sub-steps of one interval from unit h0 (sim-org/driver.cl:92-99).
+ * Returns steps taken; *retired set when the step rule returned zero. */
+template <int NF>
+__device__ __forceinline__ int64_t interval_f64(const RayParams &p, const StepRule &sr,
+                                                double *s, int h0, bool *retired)
+{
+	int64_t st = 0;
+	int dh;
+	*retired = false;
+	for (int h = h0; h < sr.n; h += dh) {
+		V4<double> q = {s[0], s[1], s[2], s[3]};
+		V4<double> u = {s[4], s[5], s[6], s[7]};
+		double x = getdt(p, q, u, sr.dts) / sr.dt * (double)sr.n;
+		dh = (x == x) ? (int)x : 0;
+		if (!dh) {
+			*retired = true;
+			break;
+		}
+		if (dh > sr.n - h)
+			dh = sr.n - h;
+		rk4_f64<NF>(p, s, (double)dh * sr.dt / (double)sr.n);
+		++st;
+	}
+	return st;
+}
+
+template <int NF>
+__device__ __forceinline__ void icond_ray(const RayParams &p, size_t g, double *s)
+{
+	size_t gi = g % (size_t)p.w_rays, gj = g / (size_t)p.w_rays;
+	double alpha = (((double)gi + 0.5) / (double)p.w_rays - 0.5) * p.w_img;
+	double beta  = (((double)gj + 0.5) / (double)p.h_rays - 0.5) * p.h_img;
+	V4<double> q, u;
+	gr_icond(p.a, p.r_obs, p.si, p.ci, p.sj, p.cj, alpha, beta, q, u);
+	s[0] = q.t; s[1] = q.x; s[2] = q.y; s[3] = q.z;
+	s[4] = u.t; s[5] = u.x; s[6] = u.y; s[7] = u.z;
+#pragma unroll
+	for (int i = 0; i < 2 * NF; ++i)
+		s[8 + i] = 0.0;
+}
+
+} // namespace gray
