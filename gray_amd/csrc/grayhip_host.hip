@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "launch.h"
+#include "spacetime.h"
 
 using namespace gray;
 
@@ -33,6 +34,7 @@ struct grayhip_ctx {
 	hipEvent_t ev[4];
 	bool have_icond;
 	int pool_blocks;
+	grayhip_spacetime *st;  /* numerical spacetime, or NULL (analytic Kerr) */
 	int64_t last_fallback;
 	Counters last_ctr;
 	double kern_ns;
@@ -57,6 +59,8 @@ static int fail(grayhip_ctx *c, int code, const char *fmt, ...)
 		if (e_ != hipSuccess)                                                     \
 			return fail(ctx, GRAYHIP_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
 	} while (0)
+
+static GridDesc grid_view(const grayhip_ctx *c);
 
 /* Prograde ISCO (Bardeen, Press & Teukolsky 1972), M = 1. */
 static double r_isco(double a)
@@ -128,6 +132,10 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 		return fail(nullptr, GRAYHIP_EINVAL, "unknown escape rule %d", cfg->escape);
 	if (cfg->fluid < GRAYHIP_FLUID_NONE || cfg->fluid > GRAYHIP_FLUID_UNIFORM)
 		return fail(nullptr, GRAYHIP_EINVAL, "unknown fluid model %d", cfg->fluid);
+	if (cfg->spacetime && cfg->n_freq > 0)
+		return fail(nullptr, GRAYHIP_EINVAL, "radiative transfer on a numerical spacetime is not supported");
+	if (cfg->spacetime && cfg->spacetime->desc.device != cfg->device)
+		return fail(nullptr, GRAYHIP_EINVAL, "spacetime lives on another device");
 	if (cfg->n_freq > 0 && cfg->fluid == GRAYHIP_FLUID_NONE)
 		return fail(nullptr, GRAYHIP_EINVAL, "radiative transfer needs a fluid model");
 	for (int i = 0; i < cfg->n_freq; ++i)
@@ -162,6 +170,7 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 		p.nu[i] = cfg->nu[i];
 	p.w_rays = (int)cfg->w_rays;
 	p.h_rays = (int)cfg->h_rays;
+	c->st = cfg->spacetime;
 
 	int rc;
 	hipError_t e;
@@ -233,9 +242,50 @@ extern "C" int grayhip_icond(grayhip_ctx *c, double w_img, double h_img, double 
 	c->p.sj = sin(d2r * j_obs);
 	c->p.cj = cos(d2r * j_obs);
 	HCK(c, hipSetDevice(c->cfg.device));
-	HCK(c, launch_icond(c->p, c->d_data, c->d_info, c->d_steps, c->n_rays, c->stream));
+	if (c->st)
+		HCK(c, launch_grid_icond(grid_view(c), c->p, c->d_data, c->d_info, c->d_steps, c->n_rays,
+		                         c->stream));
+	else
+		HCK(c, launch_icond(c->p, c->d_data, c->d_info, c->d_steps, c->n_rays, c->stream));
 	HCK(c, hipStreamSynchronize(c->stream));
 	c->have_icond = true;
+	return GRAYHIP_OK;
+}
+
+static GridDesc grid_view(const grayhip_ctx *c)
+{
+	return spacetime_view(c->st, c->cfg.t_init, c->cfg.raw_icond,
+	                      c->cfg.escape == GRAYHIP_ESC_REF);
+}
+
+/* Numerical spacetime: one launch per interval in either form; the FP32
+ * form's diagnostics are recomputed in FP64 at the end. */
+static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
+{
+	const GridDesc g = grid_view(c);
+	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
+	HCK(c, hipEventRecord(c->ev[0], c->stream));
+	for (size_t d = 0; d < n_dump; ++d) {
+		if (c->cfg.precision == GRAYHIP_FP64)
+			HCK(c, launch_grid_evolve_f64(g, one, c->d_data, c->d_info, c->d_steps, c->n_rays,
+			                              c->stream));
+		else
+			HCK(c, launch_grid_evolve_f32(g, one, c->d_data, c->d_steps, c->n_rays,
+			                              &c->d_ctr->pool_steps, c->stream));
+	}
+	HCK(c, hipEventRecord(c->ev[1], c->stream));
+	if (c->cfg.precision != GRAYHIP_FP64)
+		HCK(c, launch_grid_info_f64(g, c->d_data, c->d_info, c->n_rays, c->stream));
+	HCK(c, hipMemcpyAsync(&c->last_ctr, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost,
+	                      c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	float ms = 0.f;
+	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+	c->kern_ns = 1e6 * (double)ms;
+	c->kern_launches = (int)n_dump;
+	c->last_fallback = 0;
+	if (ns)
+		*ns = 1e6 * (double)ms;
 	return GRAYHIP_OK;
 }
 
@@ -323,6 +373,10 @@ extern "C" int grayhip_render(grayhip_ctx *c, double dt, size_t n_sub, size_t n_
 	if (rc)
 		return rc;
 	HCK(c, hipSetDevice(c->cfg.device));
+	if (c->st) {
+		sr.n_dump = 1;
+		return run_grid(c, sr, n_dump, ns);
+	}
 	if (c->cfg.precision == GRAYHIP_FP64) {
 		sr.n_dump = 1;
 		return run_fp64(c, sr, n_dump, ns);
@@ -376,7 +430,10 @@ extern "C" int grayhip_write(grayhip_ctx *c, const void *data)
 	HCK(c, hipMemcpyAsync(c->d_tmp, data, c->n_rays * (size_t)c->nd * sz, hipMemcpyHostToDevice,
 	                      c->stream));
 	HCK(c, launch_aos_to_soa(c->d_tmp, c->d_data, sz, c->n_rays, c->nd, c->stream));
-	HCK(c, launch_info_f64(c->p, c->d_data, c->d_info, c->n_rays, c->stream));
+	if (c->st)
+		HCK(c, launch_grid_info_f64(grid_view(c), c->d_data, c->d_info, c->n_rays, c->stream));
+	else
+		HCK(c, launch_info_f64(c->p, c->d_data, c->d_info, c->n_rays, c->stream));
 	HCK(c, hipMemsetAsync(c->d_steps, 0, c->n_rays * sizeof(int64_t), c->stream));
 	HCK(c, hipStreamSynchronize(c->stream));
 	c->have_icond = true;

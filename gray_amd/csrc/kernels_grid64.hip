@@ -1,0 +1,289 @@
+/*
+ * kernels_grid64.hip -- plain FP64 form of the numerical-spacetime path:
+ * camera initial conditions with the interpolated metric, one dump interval
+ * per launch (one lane per ray), and g(u,u) diagnostics.
+ *
+ * Built with -ffp-contract=off and evaluated in the order of the FP64 oracle
+ * (interpolate each Christoffel, blend in time, then contract as
+ * sim-org/dyst.cl:186-191 does), so on uniform grids it reproduces the
+ * oracle bit for bit.
+ */
+#include <climits>
+
+#include "grid_device.h"
+#include "launch.h"
+
+namespace gray {
+
+template <class S>
+__device__ __forceinline__ double ld(const S *a, uint32_t i) { return (double)a[i]; }
+
+/* Fields base..base+NC-1 of one level (interleaved, stride NCOMP) summed over
+ * the 8 corners in corner order: s = w0 v0; s = s + wk vk. */
+template <int NC, int NCOMP, class S>
+__device__ __forceinline__ void sample_lev_f64(const S *__restrict__ a, int base,
+                                               const Stencil<double> &st, double out[NC])
+{
+	{
+		const S *p = a + (size_t)st.corner(0) * NCOMP + base;
+		const double w = st.weight(0);
+#pragma unroll
+		for (int c = 0; c < NC; ++c)
+			out[c] = w * (double)p[c];
+	}
+#pragma unroll 1
+	for (int k = 1; k < 8; ++k) {
+		const S *p = a + (size_t)st.corner(k) * NCOMP + base;
+		const double w = st.weight(k);
+#pragma unroll
+		for (int c = 0; c < NC; ++c)
+			out[c] = out[c] + w * (double)p[c];
+	}
+}
+
+/* Both levels blended in time (sim-org/interp.cl:183-205). */
+template <int NC, int NCOMP, class S>
+__device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
+                                           const Stencil<double> &st, double out[NC])
+{
+	sample_lev_f64<NC, NCOMP, S>(lev[0], base, st, out);
+	if (st.wt >= 0.0) {
+		double y2[NC];
+		sample_lev_f64<NC, NCOMP, S>(lev[1], base, st, y2);
+#pragma unroll
+		for (int c = 0; c < NC; ++c)
+			out[c] = out[c] + st.wt * (y2[c] - out[c]);
+	}
+}
+
+template <class S>
+__device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double *s, double *du)
+{
+	Stencil<double> st;
+	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
+	const double u[4] = {s[4], s[5], s[6], s[7]};
+	double uGu[4];
+#pragma unroll 1
+	for (int mu = 0; mu < 4; ++mu) {
+		double g[10], v[4];
+		sample_f64<10, 40, S>(G.gam, 10 * mu, st, g);
+#pragma unroll
+		for (int a = 0; a < 4; ++a)
+			v[a] = ((g[sym(a, 0)] * u[0] + g[sym(a, 1)] * u[1]) + g[sym(a, 2)] * u[2]) +
+			       g[sym(a, 3)] * u[3];
+		const double r = ((u[0] * v[0] + u[1] * v[1]) + u[2] * v[2]) + u[3] * v[3];
+		uGu[0] = mu == 0 ? r : uGu[0];
+		uGu[1] = mu == 1 ? r : uGu[1];
+		uGu[2] = mu == 2 ? r : uGu[2];
+		uGu[3] = mu == 3 ? r : uGu[3];
+	}
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+		du[mu] = -uGu[mu] + uGu[0] * u[mu];
+}
+
+template <class S>
+__device__ __forceinline__ double grid_getuu_f64(const GridView<S> &G, const double *s)
+{
+	Stencil<double> st;
+	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
+	double m[10];
+	sample_f64<10, 10, S>(G.met, 0, st, m);
+	const double u[4] = {s[4], s[5], s[6], s[7]};
+	double k[4];
+#pragma unroll
+	for (int a = 0; a < 4; ++a)
+		k[a] = ((m[sym(a, 0)] * u[0] + m[sym(a, 1)] * u[1]) + m[sym(a, 2)] * u[2]) +
+		       m[sym(a, 3)] * u[3];
+	return ((u[0] * k[0] + u[1] * k[1]) + u[2] * k[2]) + u[3] * k[3];
+}
+
+/* sim-org/RK4.cl:37-47 on the geodesic state. */
+template <class S>
+__device__ __forceinline__ void grid_rk4_f64(const GridView<S> &G, double *s, double dt)
+{
+	double k1[8], k2[8], k[8], x[8];
+	const double hdt = 0.5 * dt;
+#define GRHS(in, out) do { out[0] = in[4]; out[1] = in[5]; out[2] = in[6]; out[3] = in[7]; \
+		grid_rhs_f64<S>(G, in, out + 4); } while (0)
+	GRHS(s, k1);
+#pragma unroll
+	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k1[i];
+	GRHS(x, k2);
+#pragma unroll
+	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k2[i];
+	GRHS(x, k);
+#pragma unroll
+	for (int i = 0; i < 8; ++i) {
+		k2[i] = k2[i] + k[i];
+		x[i] = s[i] + dt * k[i];
+	}
+	GRHS(x, k);
+#undef GRHS
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k2[i]) + k[i])) / 6.0;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_grid_icond(GridView<S> G, RayParams p,
+                                                    double *__restrict__ data,
+                                                    double *__restrict__ info,
+                                                    int64_t *__restrict__ steps, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	size_t gi = g % (size_t)p.w_rays, gj = g / (size_t)p.w_rays;
+	double alpha = (((double)gi + 0.5) / (double)p.w_rays - 0.5) * p.w_img;
+	double beta  = (((double)gj + 0.5) / (double)p.h_rays - 0.5) * p.h_img;
+	double R0 = p.r_obs * p.si - beta * p.ci;
+	double z  = p.r_obs * p.ci + beta * p.si;
+	double y  = R0 * p.sj + alpha * p.cj;
+	double x  = R0 * p.cj - alpha * p.sj;
+	double s[8] = {0.0, x, y, z, 1.0, p.si * p.cj, p.si * p.sj, p.ci};
+	if (!G.raw_icond) {
+		Stencil<double> st;
+		make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
+		double m[10];
+		sample_f64<10, 10, S>(G.met, 0, st, m);
+		double A = m[0];
+		double B = ((m[1] * s[5] + m[2] * s[6]) + m[3] * s[7]) * 2.0;
+		double C = ((((m[4] * s[5] + m[5] * s[6]) + m[6] * s[7]) * s[5]) +
+		            (((m[5] * s[5] + m[7] * s[6]) + m[8] * s[7]) * s[6])) +
+		            (((m[6] * s[5] + m[8] * s[6]) + m[9] * s[7]) * s[7]);
+		double sc = (-(B + sqrt(B * B - (4.0 * A) * C))) / (2.0 * A);
+		s[5] = s[5] / sc;
+		s[6] = s[6] / sc;
+		s[7] = s[7] / sc;
+	}
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		data[(size_t)i * n_rays + g] = s[i];
+	info[g] = grid_getuu_f64<S>(G, s);
+	steps[g] = 0;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_grid_evolve_f64(GridView<S> G, StepRule sr,
+                                                         double *__restrict__ data,
+                                                         double *__restrict__ info,
+                                                         int64_t *__restrict__ steps, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	double s[8];
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		s[i] = data[(size_t)i * n_rays + g];
+	int64_t nst = 0;
+	int dh;
+	for (int h = 0; h < sr.n; h += dh) {
+		double x = grid_getdt<double, S>(G, s[1], s[2], s[3], sr.dts) / sr.dt * (double)sr.n;
+		dh = (x == x) ? (int)x : 0;
+		if (!dh)
+			break;
+		if (dh > sr.n - h)
+			dh = sr.n - h;
+		grid_rk4_f64<S>(G, s, (double)dh * sr.dt / (double)sr.n);
+		++nst;
+	}
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		data[(size_t)i * n_rays + g] = s[i];
+	info[g] = grid_getuu_f64<S>(G, s);
+	steps[g] += nst;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_grid_info_f64(GridView<S> G, const double *__restrict__ data,
+                                                       double *__restrict__ info, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	double s[8];
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		s[i] = data[(size_t)i * n_rays + g];
+	info[g] = grid_getuu_f64<S>(G, s);
+}
+
+/* Sample of one field at given events (test and tool helper). */
+template <class S>
+__global__ void k_grid_sample_f64(GridView<S> G, const double *__restrict__ ev, int which, int comp,
+                                  double *__restrict__ out, int n)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	Stencil<double> st;
+	make_stencil<double, S>(G, ev[4 * i], ev[4 * i + 1], ev[4 * i + 2], ev[4 * i + 3], st);
+	double v[1];
+	if (which == 0)
+		sample_f64<1, 40, S>(G.gam, comp, st, v);
+	else
+		sample_f64<1, 10, S>(G.met, comp, st, v);
+	out[i] = v[0];
+}
+
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *data, double *info,
+                             int64_t *steps, size_t n_rays, hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL(k_grid_icond<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<float>(), p, data, info, steps, n_rays);
+	else
+		hipLaunchKernelGGL(k_grid_icond<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<double>(), p, data, info, steps, n_rays);
+	return hipGetLastError();
+}
+
+hipError_t launch_grid_evolve_f64(const GridDesc &gd, const StepRule &sr, double *data, double *info,
+                                  int64_t *steps, size_t n_rays, hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL(k_grid_evolve_f64<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<float>(), sr, data, info, steps, n_rays);
+	else
+		hipLaunchKernelGGL(k_grid_evolve_f64<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<double>(), sr, data, info, steps, n_rays);
+	return hipGetLastError();
+}
+
+hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *info, size_t n_rays,
+                                hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL(k_grid_info_f64<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<float>(), data, info, n_rays);
+	else
+		hipLaunchKernelGGL(k_grid_info_f64<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<double>(), data, info, n_rays);
+	return hipGetLastError();
+}
+
+hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,
+                                  double *out, int n, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL(k_grid_sample_f64<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+		                   gd.view<float>(), ev, which, comp, out, n);
+	else
+		hipLaunchKernelGGL(k_grid_sample_f64<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+		                   gd.view<double>(), ev, which, comp, out, n);
+	return hipGetLastError();
+}
+
+} // namespace gray

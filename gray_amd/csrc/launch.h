@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "grayhip_params.h"
+#include "grid_device.h"
 
 namespace gray {
 
@@ -29,5 +30,17 @@ hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
                            int64_t *steps, size_t n_rays, Counters *ctr, Fallback *fb,
                            unsigned n_blocks, hipStream_t st);
+
+hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *data, double *info,
+                             int64_t *steps, size_t n_rays, hipStream_t st);
+hipError_t launch_grid_evolve_f64(const GridDesc &gd, const StepRule &sr, double *data, double *info,
+                                  int64_t *steps, size_t n_rays, hipStream_t st);
+hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *info, size_t n_rays,
+                                hipStream_t st);
+hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,
+                                  double *out, int n, hipStream_t st);
+hipError_t launch_grid_evolve_f32(const GridDesc &gd, const StepRule &sr, double *data,
+                                  int64_t *steps, size_t n_rays, unsigned long long *nsteps,
+                                  hipStream_t st);
 
 } // namespace gray

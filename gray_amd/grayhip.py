@@ -23,7 +23,22 @@ EXPORTS = (
     "grayhip_read_steps", "grayhip_stats", "grayhip_pass_steps", "grayhip_kernel_time",
     "grayhip_n_data",
     "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
+    "grayhip_spacetime_create", "grayhip_spacetime_destroy", "grayhip_spacetime_upload",
+    "grayhip_spacetime_shift", "grayhip_spacetime_set_frozen", "grayhip_spacetime_bytes",
+    "grayhip_spacetime_sample", "grayhip_spacetime_error",
 )
+MAP_UNIFORM, MAP_FISHEYE = 0, 1
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+        ("map", C.c_int),
+        ("dtype", C.c_int),
+        ("device", C.c_int),
+    ]
 
 
 class Config(C.Structure):
@@ -39,6 +54,9 @@ class Config(C.Structure):
         ("r_esc", C.c_double),
         ("eps_stop", C.c_double),
         ("device", C.c_int),
+        ("spacetime", C.c_void_p),
+        ("t_init", C.c_double),
+        ("raw_icond", C.c_int),
     ]
 
 
@@ -77,6 +95,17 @@ def lib():
     L.grayhip_global_error.argtypes = []
     L.grayhip_global_error.restype = C.c_char_p
     L.grayhip_device_count.argtypes = []
+    L.grayhip_spacetime_create.argtypes = [C.POINTER(GridDesc), C.POINTER(vp)]
+    L.grayhip_spacetime_destroy.argtypes = [vp]
+    L.grayhip_spacetime_destroy.restype = None
+    L.grayhip_spacetime_upload.argtypes = [vp, C.c_int, C.c_double, vp, vp]
+    L.grayhip_spacetime_shift.argtypes = [vp, C.c_int]
+    L.grayhip_spacetime_set_frozen.argtypes = [vp, C.c_int]
+    L.grayhip_spacetime_bytes.argtypes = [vp]
+    L.grayhip_spacetime_bytes.restype = C.c_size_t
+    L.grayhip_spacetime_sample.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, vp]
+    L.grayhip_spacetime_error.argtypes = [vp]
+    L.grayhip_spacetime_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -89,11 +118,77 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
+class Spacetime:
+    """A numerical spacetime resident on the GPU (two time-level slots)."""
+
+    def __init__(self, n, lo, hi, map=MAP_UNIFORM, dtype=4, device=0):
+        d = GridDesc()
+        for a in range(3):
+            d.n[a], d.lo[a], d.hi[a] = n[a], lo[a], hi[a]
+        d.map, d.dtype, d.device = map, dtype, device
+        self.desc = d
+        self.n_points = n[0] * n[1] * n[2]
+        self.np_dtype = np.float32 if dtype == 4 else np.float64
+        self._h = C.c_void_p()
+        rc = lib().grayhip_spacetime_create(C.byref(d), C.byref(self._h))
+        if rc != 0:
+            raise GrayHipError(f"grayhip_spacetime_create: {rc}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GrayHipError(f"{what}: {rc}: {lib().grayhip_spacetime_error(self._h).decode()}")
+
+    def upload(self, slot, time, gam, met):
+        gam = np.ascontiguousarray(gam, dtype=self.np_dtype)
+        met = np.ascontiguousarray(met, dtype=self.np_dtype)
+        assert gam.size == 40 * self.n_points and met.size == 10 * self.n_points
+        self._check(lib().grayhip_spacetime_upload(self._h, slot, time, _ptr(gam), _ptr(met)),
+                    "upload")
+
+    def shift(self, to_t1=True):
+        self._check(lib().grayhip_spacetime_shift(self._h, int(to_t1)), "shift")
+
+    def set_frozen(self, frozen):
+        self._check(lib().grayhip_spacetime_set_frozen(self._h, int(frozen)), "set_frozen")
+
+    def bytes(self):
+        return lib().grayhip_spacetime_bytes(self._h)
+
+    def sample(self, events, which, comp, t_init=0.0):
+        ev = np.ascontiguousarray(events, dtype=np.float64).reshape(-1, 4)
+        out = np.empty(len(ev))
+        self._check(lib().grayhip_spacetime_sample(self._h, len(ev), _ptr(ev), which, comp,
+                                                   t_init, _ptr(out)), "sample")
+        return out
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().grayhip_spacetime_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Render:
     """One render on the GPU: W*H rays of one camera, one spacetime."""
 
-    def __init__(self, w_rays, h_rays, a_spin, precision=FP64, nu=(), fluid=None,
-                 escape=ESC_OUTWARD, r_esc=1000.0, eps_stop=0.0, device=0):
+    def __init__(self, w_rays, h_rays, a_spin=0.0, precision=FP64, nu=(), fluid=None,
+                 escape=ESC_OUTWARD, r_esc=1000.0, eps_stop=0.0, device=0,
+                 spacetime=None, t_init=0.0, raw_icond=False):
         cfg = Config()
         cfg.w_rays, cfg.h_rays = w_rays, h_rays
         cfg.precision = precision
@@ -108,6 +203,10 @@ class Render:
         cfg.r_esc = r_esc
         cfg.eps_stop = eps_stop
         cfg.device = device
+        cfg.spacetime = spacetime.handle if spacetime is not None else None
+        cfg.t_init = t_init
+        cfg.raw_icond = int(raw_icond)
+        self._spacetime = spacetime  # keep it alive while the render exists
         self.cfg = cfg
         self.n_rays = w_rays * h_rays
         self.n_data = 8 + 2 * len(nu)

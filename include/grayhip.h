@@ -55,6 +55,26 @@ enum {
 	GRAYHIP_FLUID_UNIFORM = 2   /* n = 1, shift = 1 (slab closed forms) */
 };
 
+/* Grid coordinate maps of a numerical spacetime (sim-org/interp.cl:69-121
+ * hard-codes the fisheye one; uniform axes are the other case in use). */
+enum {
+	GRAYHIP_MAP_UNIFORM = 0,  /* x_i = lo + i (hi - lo)/(n - 1)           */
+	GRAYHIP_MAP_FISHEYE = 1   /* x_i = sinh((A i + B)^3), A, B from lo, hi */
+};
+
+/* A numerical spacetime resident on one GPU: two time-level slots of
+ * 40 Christoffels + 10 metric components per grid point. */
+typedef struct grayhip_spacetime grayhip_spacetime;
+
+typedef struct {
+	int    n[3];             /* points along x, y, z                         */
+	double lo[3], hi[3];     /* first and last axis values (the bounding box,
+	                            sim-org/io.c:266-274)                        */
+	int    map;              /* GRAYHIP_MAP_*                                */
+	int    dtype;            /* bytes per stored real: 4 or 8                */
+	int    device;
+} grayhip_grid_desc;
+
 typedef struct {
 	size_t w_rays, h_rays;   /* W, H (sim-org/param.opts:10-11)            */
 	int    precision;        /* GRAYHIP_FP64 or GRAYHIP_FP32               */
@@ -67,6 +87,12 @@ typedef struct {
 	double eps_stop;         /* horizon stop r - r_H < eps_stop; <= 0 picks
 	                            0.01 r_H for ESC_OUTWARD, 0.01 for ESC_REF  */
 	int    device;           /* HIP device ordinal                          */
+	/* numerical spacetime (coordinates=dyst); NULL: analytic Kerr-Schild.
+	 * With a spacetime, n_freq must be 0, `escape` selects box-exit
+	 * termination (ESC_OUTWARD) or none (ESC_REF, as sim-org/dyst.cl). */
+	grayhip_spacetime *spacetime;
+	double t_init;           /* grid time of the rays' q^t = 0              */
+	int    raw_icond;        /* 1: no null normalisation (dyst.cl:89-110)   */
 } grayhip_config;
 
 typedef struct grayhip_ctx grayhip_ctx;
@@ -127,6 +153,32 @@ int  grayhip_pass_steps(grayhip_ctx *ctx, int64_t *pool_steps,
 /* Device time (ns) of the dominant kernel in the last call, measured with
  * HIP events on the stream it ran on; *launches receives its launch count. */
 int  grayhip_kernel_time(grayhip_ctx *ctx, double *ns, int *launches);
+
+/* ---- numerical spacetimes (replace the 2 x 51 image3d_t objects of
+ * sim-org/gray.h:92-93 and their loaders, io.c:302-533) ---- */
+
+/* Allocate both level slots on the device. */
+int  grayhip_spacetime_create(const grayhip_grid_desc *d, grayhip_spacetime **out);
+void grayhip_spacetime_destroy(grayhip_spacetime *st);
+/* Copy one time level (host arrays of d->dtype reals, interleaved per grid
+ * point p = (k*ny + j)*nx + i: gam[p*40 + 10*mu + c] with c in
+ * tt tx ty tz xx xy xz yy yz zz, met[p*10 + c] in the same order) into slot
+ * 0 (t1) or 1 (t2) at the given time.  Blocks until the copy is done. */
+int  grayhip_spacetime_upload(grayhip_spacetime *st, int slot, double time,
+                              const void *gam, const void *met);
+/* Slide the window: to_t1 != 0 makes the t2 slot the new t1 slot, else the
+ * t1 slot the new t2 slot (sim-org/io.c:279-300); the other slot is then
+ * free for the next upload. */
+int  grayhip_spacetime_shift(grayhip_spacetime *st, int to_t1);
+/* frozen != 0: space-only interpolation from slot 0 (sim-org/gray.c:210-228). */
+int  grayhip_spacetime_set_frozen(grayhip_spacetime *st, int frozen);
+/* Device bytes held by the spacetime. */
+size_t grayhip_spacetime_bytes(const grayhip_spacetime *st);
+/* Interpolate one field at n events (t,x,y,z) on the device in FP64:
+ * which 0: Christoffel component comp < 40, 1: metric comp < 10. */
+int  grayhip_spacetime_sample(grayhip_spacetime *st, int n, const double *events,
+                              int which, int comp, double t_init, double *out);
+const char *grayhip_spacetime_error(const grayhip_spacetime *st);
 
 int  grayhip_n_data(const grayhip_ctx *ctx);
 const char *grayhip_last_error(const grayhip_ctx *ctx);

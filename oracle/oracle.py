@@ -209,3 +209,120 @@ def evolve_subset(p, data, stride, dt, n_sub, n_dump, n_threads=0):
     assert data.flags.c_contiguous and data.dtype == np.float64
     return lib().og_evolve_subset(C.byref(p), data.shape[0], stride, _dp(data), dt,
                                   n_sub, n_dump, n_threads)
+
+
+# ---------------------------------------------------------------------------
+# numerical-spacetime oracle (grid_oracle.c)
+
+MAP_UNIFORM, MAP_FISHEYE = 0, 1
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+        ("map", C.c_int),
+        ("t1", C.c_double),
+        ("t2", C.c_double),
+        ("t_off", C.c_double),
+        ("raw_icond", C.c_int),
+        ("ref_rule", C.c_int),
+        ("gam", C.c_void_p * 2),
+        ("met", C.c_void_p * 2),
+    ]
+
+
+_grid_set = False
+
+
+def _glib():
+    global _grid_set
+    L = lib()
+    if not _grid_set:
+        gp, dp = C.POINTER(Grid), C.POINTER(C.c_double)
+        L.og_grid_down.argtypes = [gp, dp, dp, dp]
+        L.og_grid_getuu.argtypes = [gp, dp, dp]
+        L.og_grid_getuu.restype = C.c_double
+        L.og_grid_rhs.argtypes = [gp, dp, dp, dp]
+        L.og_grid_icond.argtypes = [gp] + [C.c_double] * 7 + [dp, dp]
+        L.og_grid_getdt.argtypes = [gp, dp, C.c_double]
+        L.og_grid_getdt.restype = C.c_double
+        L.og_grid_integrate.argtypes = [gp, dp, C.c_double]
+        L.og_grid_evolve_ray.argtypes = [gp, dp, C.c_double, C.c_size_t, C.POINTER(C.c_int)]
+        L.og_grid_evolve_ray.restype = C.c_int64
+        L.og_grid_icond_all.argtypes = [gp, C.c_size_t, C.c_size_t] + [C.c_double] * 7 + [dp, dp]
+        L.og_grid_evolve_all.argtypes = [gp, C.c_size_t, dp, dp, C.c_double, C.c_size_t,
+                                         C.POINTER(C.c_int64), C.c_int]
+        L.og_grid_evolve_all.restype = C.c_int64
+        L.og_grid_sample.argtypes = [gp, C.c_int, C.c_int, dp]
+        L.og_grid_sample.restype = C.c_double
+        _grid_set = True
+    return L
+
+
+class GridSpacetime:
+    """Oracle view of a gridded spacetime: interleaved FP64 arrays
+    gam[level] (n_points x 40) and met[level] (n_points x 10)."""
+
+    def __init__(self, n, lo, hi, gam, met, times=(0.0, 0.0), map=MAP_UNIFORM, t_off=0.0,
+                 raw_icond=False, ref_rule=False):
+        g = Grid()
+        for a in range(3):
+            g.n[a], g.lo[a], g.hi[a] = n[a], lo[a], hi[a]
+        g.map = map
+        g.t1, g.t2 = times
+        g.t_off = t_off
+        g.raw_icond, g.ref_rule = int(raw_icond), int(ref_rule)
+        self._keep = []
+        for l in range(2):
+            ga = np.ascontiguousarray(gam[min(l, len(gam) - 1)], dtype=np.float64)
+            me = np.ascontiguousarray(met[min(l, len(met) - 1)], dtype=np.float64)
+            self._keep += [ga, me]
+            g.gam[l], g.met[l] = ga.ctypes.data, me.ctypes.data
+        self.g = g
+
+    def sample(self, which, comp, q):
+        return _glib().og_grid_sample(C.byref(self.g), which, comp, _dp(_vec(q)))
+
+    def rhs(self, q, u):
+        du = np.zeros(4)
+        _glib().og_grid_rhs(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)), _dp(du))
+        return du
+
+    def down(self, q, u):
+        k = np.zeros(4)
+        _glib().og_grid_down(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)), _dp(k))
+        return k
+
+    def getuu(self, q, u):
+        return _glib().og_grid_getuu(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)))
+
+    def getdt(self, q, dt):
+        return _glib().og_grid_getdt(C.byref(self.g), _dp(_vec(q)), dt)
+
+    def icond(self, r_obs, i_obs, j_obs, alpha, beta):
+        si, ci, sj, cj = camera_trig(i_obs, j_obs)
+        q, u = np.zeros(4), np.zeros(4)
+        _glib().og_grid_icond(C.byref(self.g), r_obs, si, ci, sj, cj, alpha, beta, _dp(q), _dp(u))
+        return q, u
+
+    def icond_all(self, w, h, w_img, h_img, r_obs, i_obs, j_obs):
+        si, ci, sj, cj = camera_trig(i_obs, j_obs)
+        data, info = np.zeros((w * h, 8)), np.zeros(w * h)
+        _glib().og_grid_icond_all(C.byref(self.g), w, h, w_img, h_img, r_obs, si, ci, sj, cj,
+                                  _dp(data), _dp(info))
+        return data, info
+
+    def evolve_all(self, data, info, dt, n_sub, steps=None, n_threads=0):
+        sp = steps.ctypes.data_as(C.POINTER(C.c_int64)) if steps is not None else None
+        return _glib().og_grid_evolve_all(C.byref(self.g), data.shape[0], _dp(data), _dp(info),
+                                          dt, n_sub, sp, n_threads)
+
+    def render(self, w, h, w_img, h_img, r_obs, i_obs, j_obs, dt=-32.0, n_sub=1024, n_dump=1,
+               n_threads=0):
+        data, info = self.icond_all(w, h, w_img, h_img, r_obs, i_obs, j_obs)
+        steps = np.zeros(w * h, dtype=np.int64)
+        for _ in range(n_dump):
+            self.evolve_all(data, info, dt, n_sub, steps, n_threads)
+        return data, info, steps

@@ -1,0 +1,152 @@
+"""GPU parity of the numerical-spacetime path (config-3 physics at small
+sizes) against the FP64 grid oracle, through the C ABI.
+
+Bar: the plain FP64 form on uniform grids evaluates the oracle's sequence
+and must match it bit for bit (state and step counts), for FP64 and FP32
+grid storage (FP32 values are converted exactly).  Fisheye axes go through
+cbrt/asinh/sinh, which are not correctly rounded: 1e-10 relative there.
+The FP32 form is compared with the FP64 form.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gray_amd import grayhip as G
+from gray_amd import job as J
+from gray_amd import spacetime as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOX = 40.0
+
+
+def make_grid(n, dtype, times=(0.0,), fisheye=False, a=0.7, v=0.3):
+    xs = S.axis(-BOX, BOX, n, fisheye)
+    b = S.bks(1.0, a, v, (1, 0, 0))
+    lv = [S.bks_sample(b, t, xs, xs, xs) for t in times]
+    t = np.float32 if dtype == 4 else np.float64
+    lv = [(g.astype(t), m.astype(t)) for g, m in lv]
+    return xs, lv
+
+
+def device_grid(n, dtype, lv, times, fisheye=False):
+    st = G.Spacetime((n,) * 3, (-BOX,) * 3, (BOX,) * 3,
+                     map=G.MAP_FISHEYE if fisheye else G.MAP_UNIFORM, dtype=dtype)
+    for slot, (g, m) in enumerate(lv):
+        st.upload(slot, times[slot], g, m)
+    st.set_frozen(len(lv) == 1)
+    return st
+
+
+def oracle_grid(n, lv, times, t_off=0.0, fisheye=False):
+    return O.GridSpacetime((n,) * 3, (-BOX,) * 3, (BOX,) * 3,
+                           [g.astype(np.float64) for g, _ in lv],
+                           [m.astype(np.float64) for _, m in lv],
+                           times=(times[0], times[-1]), t_off=t_off,
+                           map=O.MAP_FISHEYE if fisheye else O.MAP_UNIFORM)
+
+
+@pytest.mark.parametrize("dtype", [4, 8])
+def test_sample_bitwise(dtype):
+    n = 17
+    xs, lv = make_grid(n, dtype, times=(0.0, 1.0))
+    st = device_grid(n, dtype, lv, (0.0, 1.0))
+    og = oracle_grid(n, lv, (0.0, 1.0), t_off=0.25)
+    rng = np.random.default_rng(2)
+    ev = np.column_stack([rng.uniform(-1, 2, 200), rng.uniform(-45, 45, (200, 3))])
+    for which, comp in [(0, 0), (0, 13), (0, 39), (1, 0), (1, 9)]:
+        got = st.sample(ev, which, comp, t_init=0.25)
+        want = np.array([og.sample(which, comp, e) for e in ev])
+        assert np.array_equal(got, want), (which, comp, np.abs(got - want).max())
+
+
+def render_pair(n, dtype, times, w=12, h=12, n_dump=2, precision=G.FP64, fisheye=False,
+                t_init=0.0):
+    xs, lv = make_grid(n, dtype, times=times, fisheye=fisheye)
+    st = device_grid(n, dtype, lv, times, fisheye)
+    with G.Render(w, h, precision=precision, spacetime=st, t_init=t_init) as r:
+        r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+        for _ in range(n_dump):
+            r.evolve(-32.0, 1024)
+        d1, i1 = r.read_f64()
+        s1 = r.steps()
+    og = oracle_grid(n, lv, times, t_off=t_init, fisheye=fisheye)
+    d0, i0, s0 = og.render(w, h, 32.0, 32.0, 35.0, 45.0, 0.0, n_dump=n_dump)
+    st.close()
+    return (d0, i0, s0), (d1, i1, s1)
+
+
+@pytest.mark.parametrize("dtype", [4, 8])
+def test_render_fp64_frozen_bitwise(dtype):
+    (d0, i0, s0), (d1, i1, s1) = render_pair(33, dtype, (0.0,))
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1), np.abs(d0 - d1).max()
+    assert np.array_equal(i0, i1)
+
+
+def test_render_fp64_two_levels_bitwise():
+    (d0, i0, s0), (d1, i1, s1) = render_pair(33, 4, (0.0, 1.0), t_init=1.0)
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1), np.abs(d0 - d1).max()
+
+
+def test_render_fp64_fisheye_close():
+    (d0, i0, s0), (d1, i1, s1) = render_pair(33, 8, (0.0,), fisheye=True, n_dump=1)
+    assert (s0 == s1).mean() > 0.95
+    same = s0 == s1
+    rel = np.abs(d1[same] - d0[same]) / np.maximum(np.abs(d0[same]), 1e-300)
+    assert rel.max() < 1e-9, rel.max()
+
+
+def test_render_fp32_form_vs_fp64():
+    """FP32 form on an FP32 grid against the FP64 form on the FP64 grid."""
+    n, times = 33, (0.0,)
+    xs, lv64 = make_grid(n, 8, times=times)
+    st64 = device_grid(n, 8, lv64, times)
+    lv32 = [(g.astype(np.float32), m.astype(np.float32)) for g, m in lv64]
+    st32 = device_grid(n, 4, lv32, times)
+    out = []
+    for st, prec in ((st64, G.FP64), (st32, G.FP32)):
+        with G.Render(24, 24, precision=prec, spacetime=st) as r:
+            r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+            r.render(-32.0, 1024, 3)
+            out.append((r.read_f64()[0], r.steps()))
+    (d64, s64), (d32, s32) = out
+    same = s64 == s32
+    assert same.mean() > 0.9
+    dx = np.abs(d32[same, 1:4] - d64[same, 1:4]).max(axis=1)
+    assert np.median(dx) < 1e-3 and dx.max() < 0.5, (np.median(dx), dx.max())
+
+
+def test_launcher_dyst_job_matches_oracle(tmp_path):
+    exe = os.path.join(os.path.dirname(J.LIB_PATH), "gray-mkgrid")
+    f = str(tmp_path / "bks.h5")
+    r = subprocess.run([exe, f"out={f}", "n=33", "a=0.7", "v=0.3", "axis=x", "dtype=4",
+                        "times=0.0"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([J.LAUNCHER, "gray", "coor=dyst", f"data={f}", "W=8", "H=8", "w=32",
+                        "h=32", "r=35", "i=45", "p=8", "N=2", f"out={tmp_path}/%04d.raw"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "freezing spacetime" in r.stdout
+    xs, ys, zs, dt = S.h5_read_axes(f)
+    g, m = S.h5_read_level(f, "0.0", (33, 33, 33))
+    og = O.GridSpacetime((33,) * 3, (-40.0,) * 3, (40.0,) * 3, [g], [m])
+    d0, i0, s0 = og.render(8, 8, 32.0, 32.0, 35.0, 45.0, 0.0, n_dump=2)
+    states, diag = J.load_raw(f"{tmp_path}/0002.raw")
+    assert np.array_equal(states.reshape(-1, 8), d0)
+    assert np.array_equal(diag.ravel(), i0)
+
+
+def test_spacetime_errors():
+    with pytest.raises(G.GrayHipError):
+        G.Spacetime((1, 4, 4), (-1,) * 3, (1,) * 3)
+    st = G.Spacetime((4, 4, 4), (-1,) * 3, (1,) * 3)
+    with pytest.raises(G.GrayHipError):
+        st.set_frozen(False)  # only one level could be loaded
+    with pytest.raises(G.GrayHipError):
+        G.Render(4, 4, nu=(1.0,), spacetime=st)  # no transfer on grids
+    st.close()
