@@ -220,9 +220,11 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u)
 	return acc;
 }
 
-/* sim-org/phys.cl:48-64 with the selectable escape rule. */
-template <class T>
-__device__ __forceinline__ T getdt(const RayParams &p, const V4<T> &q,
+/* sim-org/phys.cl:48-64 with the selectable escape rule.  P is any
+ * parameter block with the fields a, r_h, eps_stop, escape, r_esc (the
+ * render's RayParams, or per-lane parameters of a serving pool). */
+template <class T, class P>
+__device__ __forceinline__ T getdt(const P &p, const V4<T> &q,
                                    const V4<T> &u, T dt)
 {
 	T r   = Mth<T>::sqrt_(getrr(T(p.a), q));
@@ -251,8 +253,8 @@ __device__ __forceinline__ T getdt(const RayParams &p, const V4<T> &q,
  * shift = -k.u with k = down(q, u_photon)  (sim-org/flow.cl:36-58 with the
  * fluid model replaced).  Inside r_isco nothing else is evaluated, so the
  * horizon approach pays for the geodesic only. */
-template <class T>
-__device__ __forceinline__ void flow_disk(const RayParams &p, const V4<T> &q,
+template <class T, class P>
+__device__ __forceinline__ void flow_disk(const P &p, const V4<T> &q,
                                           const V4<T> &u, T &n, T &shift)
 {
 	typedef Mth<T> M;
@@ -283,8 +285,8 @@ __device__ __forceinline__ void flow_disk(const RayParams &p, const V4<T> &q,
  * j = n nu^-1/2 and absorption alpha = 0.1 n nu^-5/2 at nu = nu_i shift:
  *   dI/dl = -j exp(-tau) / (shift^2 + 1e-28),  dtau/dl = -alpha shift.
  * Fills dI[0..NF), dtau[0..NF). */
-template <class T, int NF>
-__device__ __forceinline__ void rt_rhs(const RayParams &p, const V4<T> &q,
+template <class T, int NF, class P>
+__device__ __forceinline__ void rt_rhs(const P &p, const V4<T> &q,
                                        const V4<T> &u, const T *tau,
                                        T *dI, T *dtau)
 {

@@ -10,93 +10,10 @@
  */
 #include <climits>
 
-#include "grid_device.h"
+#include "grid_f64.h"
 #include "launch.h"
 
 namespace gray {
-
-template <class S>
-__device__ __forceinline__ double ld(const S *a, uint32_t i) { return (double)a[i]; }
-
-/* Fields base..base+NC-1 of one level (interleaved, stride NCOMP) summed over
- * the 8 corners in corner order: s = w0 v0; s = s + wk vk. */
-template <int NC, int NCOMP, class S>
-__device__ __forceinline__ void sample_lev_f64(const S *__restrict__ a, int base,
-                                               const Stencil<double> &st, double out[NC])
-{
-	{
-		const S *p = a + (size_t)st.corner(0) * NCOMP + base;
-		const double w = st.weight(0);
-#pragma unroll
-		for (int c = 0; c < NC; ++c)
-			out[c] = w * (double)p[c];
-	}
-#pragma unroll 1
-	for (int k = 1; k < 8; ++k) {
-		const S *p = a + (size_t)st.corner(k) * NCOMP + base;
-		const double w = st.weight(k);
-#pragma unroll
-		for (int c = 0; c < NC; ++c)
-			out[c] = out[c] + w * (double)p[c];
-	}
-}
-
-/* Both levels blended in time (sim-org/interp.cl:183-205). */
-template <int NC, int NCOMP, class S>
-__device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
-                                           const Stencil<double> &st, double out[NC])
-{
-	sample_lev_f64<NC, NCOMP, S>(lev[0], base, st, out);
-	if (st.wt >= 0.0) {
-		double y2[NC];
-		sample_lev_f64<NC, NCOMP, S>(lev[1], base, st, y2);
-#pragma unroll
-		for (int c = 0; c < NC; ++c)
-			out[c] = out[c] + st.wt * (y2[c] - out[c]);
-	}
-}
-
-template <class S>
-__device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double *s, double *du)
-{
-	Stencil<double> st;
-	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
-	const double u[4] = {s[4], s[5], s[6], s[7]};
-	double uGu[4];
-#pragma unroll 1
-	for (int mu = 0; mu < 4; ++mu) {
-		double g[10], v[4];
-		sample_f64<10, 40, S>(G.gam, 10 * mu, st, g);
-#pragma unroll
-		for (int a = 0; a < 4; ++a)
-			v[a] = ((g[sym(a, 0)] * u[0] + g[sym(a, 1)] * u[1]) + g[sym(a, 2)] * u[2]) +
-			       g[sym(a, 3)] * u[3];
-		const double r = ((u[0] * v[0] + u[1] * v[1]) + u[2] * v[2]) + u[3] * v[3];
-		uGu[0] = mu == 0 ? r : uGu[0];
-		uGu[1] = mu == 1 ? r : uGu[1];
-		uGu[2] = mu == 2 ? r : uGu[2];
-		uGu[3] = mu == 3 ? r : uGu[3];
-	}
-#pragma unroll
-	for (int mu = 0; mu < 4; ++mu)
-		du[mu] = -uGu[mu] + uGu[0] * u[mu];
-}
-
-template <class S>
-__device__ __forceinline__ double grid_getuu_f64(const GridView<S> &G, const double *s)
-{
-	Stencil<double> st;
-	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
-	double m[10];
-	sample_f64<10, 10, S>(G.met, 0, st, m);
-	const double u[4] = {s[4], s[5], s[6], s[7]};
-	double k[4];
-#pragma unroll
-	for (int a = 0; a < 4; ++a)
-		k[a] = ((m[sym(a, 0)] * u[0] + m[sym(a, 1)] * u[1]) + m[sym(a, 2)] * u[2]) +
-		       m[sym(a, 3)] * u[3];
-	return ((u[0] * k[0] + u[1] * k[1]) + u[2] * k[2]) + u[3] * k[3];
-}
 
 /* sim-org/RK4.cl:37-47 on the geodesic state. */
 template <class S>

@@ -10,8 +10,8 @@
 
 namespace gray {
 
-template <int NF>
-__device__ __forceinline__ void rhs_f64(const RayParams &p, const double *s, double *ds)
+template <int NF, class P>
+__device__ __forceinline__ void rhs_f64(const P &p, const double *s, double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
@@ -24,26 +24,26 @@ __device__ __forceinline__ void rhs_f64(const RayParams &p, const double *s, dou
 
 /* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at
  * s + (dt/2) k1, s + (dt/2) k2, s + dt k3. */
-template <int NF>
-__device__ __forceinline__ void rk4_f64(const RayParams &p, double *s, double dt)
+template <int NF, class P>
+__device__ __forceinline__ void rk4_f64(const P &p, double *s, double dt)
 {
 	constexpr int ND = 8 + 2 * NF;
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
 
-	rhs_f64<NF>(p, s, k1);
+	rhs_f64<NF, P>(p, s, k1);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
-	rhs_f64<NF>(p, x, k23);
+	rhs_f64<NF, P>(p, x, k23);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
-	rhs_f64<NF>(p, x, k);
+	rhs_f64<NF, P>(p, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		k23[i] = k23[i] + k[i];
 		x[i] = s[i] + dt * k[i];
 	}
-	rhs_f64<NF>(p, x, k);
+	rhs_f64<NF, P>(p, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k23[i]) + k[i])) / 6.0;
@@ -51,8 +51,8 @@ __device__ __forceinline__ void rk4_f64(const RayParams &p, double *s, double dt
 
 /* Sub-steps of one interval from unit h0 (sim-org/driver.cl:92-99).
  * Returns steps taken; *retired set when the step rule returned zero. */
-template <int NF>
-__device__ __forceinline__ int64_t interval_f64(const RayParams &p, const StepRule &sr,
+template <int NF, class P>
+__device__ __forceinline__ int64_t interval_f64(const P &p, const StepRule &sr,
                                                 double *s, int h0, bool *retired)
 {
 	int64_t st = 0;
@@ -69,14 +69,14 @@ __device__ __forceinline__ int64_t interval_f64(const RayParams &p, const StepRu
 		}
 		if (dh > sr.n - h)
 			dh = sr.n - h;
-		rk4_f64<NF>(p, s, (double)dh * sr.dt / (double)sr.n);
+		rk4_f64<NF, P>(p, s, (double)dh * sr.dt / (double)sr.n);
 		++st;
 	}
 	return st;
 }
 
-template <int NF>
-__device__ __forceinline__ void icond_ray(const RayParams &p, size_t g, double *s)
+template <int NF, class P>
+__device__ __forceinline__ void icond_ray(const P &p, size_t g, double *s)
 {
 	size_t gi = g % (size_t)p.w_rays, gj = g / (size_t)p.w_rays;
 	double alpha = (((double)gi + 0.5) / (double)p.w_rays - 0.5) * p.w_img;

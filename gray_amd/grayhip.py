@@ -288,3 +288,189 @@ class Render:
         ns, n = C.c_double(), C.c_int()
         self._check(lib().grayhip_kernel_time(self._h, C.byref(ns), C.byref(n)), "kernel_time")
         return ns.value, n.value
+
+
+# ---------------------------------------------------------------------------
+# serving pool (grayhip_server_*)
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+EXPORTS = EXPORTS + (
+    "grayhip_server_create", "grayhip_server_destroy", "grayhip_server_submit",
+    "grayhip_server_step", "grayhip_server_next_done", "grayhip_server_state",
+    "grayhip_server_result", "grayhip_server_stats", "grayhip_server_error",
+)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("slots", C.c_size_t),
+        ("hbm_budget", C.c_size_t),
+        ("epoch_steps", C.c_int),
+        ("max_requests", C.c_int),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("w_rays", C.c_size_t),
+        ("h_rays", C.c_size_t),
+        ("a_spin", C.c_double),
+        ("n_freq", C.c_int),
+        ("nu", C.c_double * 4),
+        ("fluid", C.c_int),
+        ("escape", C.c_int),
+        ("r_esc", C.c_double),
+        ("eps_stop", C.c_double),
+        ("w_img", C.c_double),
+        ("h_img", C.c_double),
+        ("r_obs", C.c_double),
+        ("i_obs", C.c_double),
+        ("j_obs", C.c_double),
+        ("dt", C.c_double),
+        ("n_sub", C.c_size_t),
+        ("n_dump", C.c_size_t),
+        ("spacetime", C.c_void_p),
+        ("t_init", C.c_double),
+    ]
+
+
+_srv_set = False
+
+
+def _slib():
+    global _srv_set
+    L = lib()
+    if not _srv_set:
+        vp = C.c_void_p
+        L.grayhip_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.grayhip_server_destroy.argtypes = [vp]
+        L.grayhip_server_destroy.restype = None
+        L.grayhip_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+        L.grayhip_server_step.argtypes = [vp, C.POINTER(C.c_int)]
+        L.grayhip_server_next_done.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.grayhip_server_state.argtypes = [vp, C.c_int64, C.POINTER(C.c_int)]
+        L.grayhip_server_result.argtypes = [vp, C.c_int64, vp, vp, vp]
+        L.grayhip_server_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.grayhip_server_error.argtypes = [vp]
+        L.grayhip_server_error.restype = C.c_char_p
+        _srv_set = True
+    return L
+
+
+def kerr_request(side, a_spin, i_obs, nu=(), j_obs=0.0, w_img=64.0, r_obs=1000.0, n_dump=128,
+                 dt=-32.0, n_sub=1024, h_side=None):
+    q = Request()
+    q.w_rays, q.h_rays = side, h_side or side
+    q.a_spin = a_spin
+    q.n_freq = len(nu)
+    for i, v in enumerate(nu):
+        q.nu[i] = v
+    q.fluid = FLUID_DISK if nu else FLUID_NONE
+    q.escape = ESC_OUTWARD
+    q.r_esc = 1000.0
+    q.w_img = q.h_img = w_img
+    q.r_obs, q.i_obs, q.j_obs = r_obs, i_obs, j_obs
+    q.dt, q.n_sub, q.n_dump = dt, n_sub, n_dump
+    return q
+
+
+def grid_request(side, spacetime, i_obs, w_img=32.0, r_obs=35.0, n_dump=4, t_init=1.0,
+                 dt=-32.0, n_sub=1024):
+    q = Request()
+    q.w_rays = q.h_rays = side
+    q.escape = ESC_OUTWARD
+    q.w_img = q.h_img = w_img
+    q.r_obs, q.i_obs = r_obs, i_obs
+    q.dt, q.n_sub, q.n_dump = dt, n_sub, n_dump
+    q.spacetime = spacetime.handle
+    q.t_init = t_init
+    return q
+
+
+class Server:
+    """The serving pool: submit requests, step epochs, collect results."""
+
+    def __init__(self, slots=1 << 16, hbm_budget=1 << 30, epoch_steps=2048, max_requests=1024,
+                 device=0):
+        c = ServerConfig(device, slots, hbm_budget, epoch_steps, max_requests)
+        self._h = C.c_void_p()
+        rc = _slib().grayhip_server_create(C.byref(c), C.byref(self._h))
+        if rc != 0:
+            raise GrayHipError(f"grayhip_server_create: {rc}: "
+                               f"{_slib().grayhip_server_error(None).decode()}")
+        self._meta = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GrayHipError(f"{what}: {rc}: {_slib().grayhip_server_error(self._h).decode()}")
+
+    def submit(self, q):
+        i = C.c_int64()
+        self._check(_slib().grayhip_server_submit(self._h, C.byref(q), C.byref(i)), "submit")
+        self._meta[i.value] = (q.w_rays * q.h_rays, 8 + 2 * q.n_freq)
+        return i.value
+
+    def step(self):
+        busy = C.c_int()
+        self._check(_slib().grayhip_server_step(self._h, C.byref(busy)), "step")
+        return bool(busy.value)
+
+    def next_done(self):
+        i = C.c_int64()
+        rc = _slib().grayhip_server_next_done(self._h, C.byref(i))
+        if rc == 1:
+            return None
+        self._check(rc, "next_done")
+        return i.value
+
+    def state(self, rid):
+        s = C.c_int()
+        self._check(_slib().grayhip_server_state(self._h, rid, C.byref(s)), "state")
+        return s.value
+
+    def result(self, rid):
+        n, nd = self._meta[rid]
+        data, info = np.empty((n, nd)), np.empty(n)
+        steps = np.empty(n, dtype=np.int64)
+        self._check(_slib().grayhip_server_result(self._h, rid, _ptr(data), _ptr(info), _ptr(steps)),
+                    "result")
+        del self._meta[rid]
+        return data, info, steps
+
+    def stats(self):
+        st = (C.c_int64 * 3)()
+        ep = C.c_int64()
+        self._check(_slib().grayhip_server_stats(self._h, st, C.byref(ep)), "stats")
+        return list(st), ep.value
+
+    def run(self, max_epochs=100000):
+        """Step until idle; returns {id: (data, info, steps)} of all results."""
+        out = {}
+        for _ in range(max_epochs):
+            busy = self.step()
+            while True:
+                rid = self.next_done()
+                if rid is None:
+                    break
+                out[rid] = self.result(rid)
+            if not busy:
+                return out
+        raise GrayHipError("server did not drain")
+
+    def close(self):
+        if self._h:
+            _slib().grayhip_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

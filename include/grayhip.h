@@ -180,6 +180,63 @@ int  grayhip_spacetime_sample(grayhip_spacetime *st, int n, const double *events
                               int which, int comp, double t_init, double *out);
 const char *grayhip_spacetime_error(const grayhip_spacetime *st);
 
+/* ---- serving: many renders of uneven size share one ray pool ----
+ *
+ * No counterpart in the reference (one job renders one image).  Requests
+ * are queued by grayhip_server_submit and admitted when their output fits
+ * in the HBM budget; each grayhip_server_step runs one epoch: finished rays
+ * free their slots, freed slots are refilled with rays of admitted
+ * requests (initial conditions in FP64 on the device), and every occupied
+ * slot advances by at most `epoch_steps` RK4 steps.  Analytic-Kerr rays run
+ * in the FP32 form and move to FP64 re-integration when their step
+ * collapses; gridded rays run in the FP32 form on resident spacetimes.  A
+ * ray's arithmetic does not depend on which slot, epoch or neighbours it
+ * gets, so a request's result is the same alone and in any mix. */
+typedef struct grayhip_server grayhip_server;
+
+typedef struct {
+	int    device;
+	size_t slots;            /* rays in flight (pool size)                  */
+	size_t hbm_budget;       /* bytes of request outputs admitted at once   */
+	int    epoch_steps;      /* RK4 steps per ray per epoch                 */
+	int    max_requests;     /* admitted requests at once                   */
+} grayhip_server_config;
+
+typedef struct {
+	size_t w_rays, h_rays;
+	double a_spin;           /* analytic Kerr (ignored with a spacetime)     */
+	int    n_freq;           /* 0..4                                         */
+	double nu[4];
+	int    fluid, escape;
+	double r_esc, eps_stop;
+	double w_img, h_img, r_obs, i_obs, j_obs;
+	double dt;               /* affine length of one dump interval           */
+	size_t n_sub, n_dump;
+	grayhip_spacetime *spacetime; /* FP32 storage; NULL: analytic Kerr       */
+	double t_init;
+} grayhip_request;
+
+enum { GRAYHIP_REQ_QUEUED = 0, GRAYHIP_REQ_RUNNING = 1, GRAYHIP_REQ_DONE = 2 };
+
+int  grayhip_server_create(const grayhip_server_config *cfg, grayhip_server **out);
+void grayhip_server_destroy(grayhip_server *srv);
+/* Queue a request; *id identifies it.  Output needs
+ * n_rays * ((8 + 2 n_freq) + 2) * 8 bytes of the budget. */
+int  grayhip_server_submit(grayhip_server *srv, const grayhip_request *req, int64_t *id);
+/* One epoch (blocks until it is done).  *busy is set to 0 when no request
+ * is queued or running. */
+int  grayhip_server_step(grayhip_server *srv, int *busy);
+/* Next finished request not yet reported (returns 0 and *id, or 1 if none). */
+int  grayhip_server_next_done(grayhip_server *srv, int64_t *id);
+int  grayhip_server_state(grayhip_server *srv, int64_t id, int *state);
+/* Copy a finished request's rays (AoS [n_rays][8 + 2 n_freq] FP64, info,
+ * RK4 steps per ray; any may be NULL) and release it. */
+int  grayhip_server_result(grayhip_server *srv, int64_t id, double *data, double *info,
+                           int64_t *steps);
+/* Totals since creation: RK4 steps (FP32 Kerr, FP64 Kerr, gridded), epochs. */
+int  grayhip_server_stats(grayhip_server *srv, int64_t steps[3], int64_t *epochs);
+const char *grayhip_server_error(const grayhip_server *srv);
+
 int  grayhip_n_data(const grayhip_ctx *ctx);
 const char *grayhip_last_error(const grayhip_ctx *ctx);
 const char *grayhip_global_error(void);
