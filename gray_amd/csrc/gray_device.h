@@ -122,10 +122,19 @@ __device__ __forceinline__ void gr_icond(T a, T r_obs, T si, T ci, T sj, T cj,
 	u.z = M::div(u.z, s);
 }
 
+/* Kerr-Schild quantities at the event that gr_rhs computes anyway: r, f and
+ * the covariant l_mu = (1, lx, ly, lz), and l.u.  With them the metric is
+ * g = eta + f l l, so k_mu = eta_mu nu u^nu + f (l.u) l_mu needs no second
+ * evaluation (used by the FP32 form's transfer). */
+template <class T> struct KsAux {
+	T r, f, lx, ly, lz, lu;
+};
+
 /* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS.
- * Returns du/dlambda (dq/dlambda = u). */
+ * Returns du/dlambda (dq/dlambda = u); fills *aux when given. */
 template <class T>
-__device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u)
+__device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
+                                        KsAux<T> *aux = nullptr)
 {
 	typedef Mth<T> M;
 	T f, dx_f, dy_f, dz_f;
@@ -140,6 +149,8 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u)
 		T dd = M::sqrt_(kk * kk + aa * zz);
 		T rr = dd + kk;
 		T r  = M::sqrt_(rr);
+		if (aux)
+			aux->r = r;
 		T ir = M::div(T(1.0), r);
 		T ss = rr + aa;
 		T iss = M::div(T(1.0), ss);
@@ -186,6 +197,13 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u)
 	{
 		V4<T> Dx, Dy, Dz;
 		T lu  = ((u.t + lx * u.x) + ly * u.y) + lz * u.z;
+		if (aux) {
+			aux->f = f;
+			aux->lx = lx;
+			aux->ly = ly;
+			aux->lz = lz;
+			aux->lu = lu;
+		}
 		T flu = f * lu;
 		Dx.t = dx_f * lu + f * ((dx_lx * u.x + dx_ly * u.y) + dx_lz * u.z);
 		Dy.t = dy_f * lu + f * ((dy_lx * u.x + dy_ly * u.y) + dy_lz * u.z);

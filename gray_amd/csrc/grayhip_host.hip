@@ -30,8 +30,8 @@ struct grayhip_ctx {
 	Counters *d_ctr;      /* counters of the FP32 pass                       */
 	Fallback *d_fb;
 	void *d_tmp;          /* AoS staging for read/write                      */
-	hipStream_t stream;
-	hipEvent_t ev[4];
+	hipStream_t stream, stream2;
+	hipEvent_t ev[8];
 	bool have_icond;
 	int pool_blocks;
 	grayhip_spacetime *st;  /* numerical spacetime, or NULL (analytic Kerr) */
@@ -97,11 +97,13 @@ static void release(grayhip_ctx *c)
 	(void)hipFree(c->d_ctr);
 	(void)hipFree(c->d_fb);
 	(void)hipFree(c->d_tmp);
-	for (int i = 0; i < 4; ++i)
+	for (int i = 0; i < 8; ++i)
 		if (c->ev[i])
 			(void)hipEventDestroy(c->ev[i]);
 	if (c->stream)
 		(void)hipStreamDestroy(c->stream);
+	if (c->stream2)
+		(void)hipStreamDestroy(c->stream2);
 	free(c);
 }
 
@@ -184,7 +186,7 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 			{(void **)&c->d_data, n * nd * sizeof(double)},
 			{(void **)&c->d_info, n * sizeof(double)},
 			{(void **)&c->d_steps, n * sizeof(int64_t)},
-			{(void **)&c->d_ctr, sizeof(Counters)},
+			{(void **)&c->d_ctr, 2 * sizeof(Counters)},
 			{(void **)&c->d_fb, n * sizeof(Fallback)},
 			{(void **)&c->d_tmp, n * nd * sizeof(double)},
 		};
@@ -196,11 +198,12 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 			}
 		}
 	}
-	if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+	if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+	    (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess) {
 		rc = fail(nullptr, GRAYHIP_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
 		goto bad;
 	}
-	for (int i = 0; i < 4; ++i)
+	for (int i = 0; i < 8; ++i)
 		if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) {
 			rc = fail(nullptr, GRAYHIP_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
 			goto bad;
@@ -304,34 +307,56 @@ static int make_rule(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, Ste
 	return GRAYHIP_OK;
 }
 
-/* FP32 form: persistent pool over all rays, then FP64 re-integration of the
- * rays it handed over. */
+/* FP32 form: two persistent pool passes -- the middle half of the image
+ * rows (the shadow, whose rays reach the horizon and collapse their step)
+ * and then the outer rows -- with the FP64 re-integration of the first
+ * pass's hand-overs on a second stream, beside the second pass; the second
+ * pass's few hand-overs follow it.  Fallback launches cover the largest
+ * possible queue and idle threads exit, so nothing waits on the host. */
 static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 {
-	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
-	HCK(c, hipEventRecord(c->ev[0], c->stream));
-	HCK(c, hipEventRecord(c->ev[2], c->stream));
-	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, c->d_ctr,
-	                       c->d_fb, (unsigned)c->pool_blocks, c->stream));
-	HCK(c, hipEventRecord(c->ev[3], c->stream));
-	Counters h;
-	HCK(c, hipMemcpyAsync(&h, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-	HCK(c, hipStreamSynchronize(c->stream));
-	const unsigned nfb = h.n_fb;
-	if (nfb > c->n_rays)
-		return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow (%u)", nfb);
-	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, c->d_fb,
-	                           c->d_ctr, nfb, c->stream));
-	HCK(c, hipEventRecord(c->ev[1], c->stream));
-	HCK(c, hipMemcpyAsync(&c->last_ctr, c->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost,
-	                      c->stream));
-	HCK(c, hipStreamSynchronize(c->stream));
-	float ms = 0.f, kms = 0.f;
+	const int W = (int)c->cfg.w_rays, H = (int)c->cfg.h_rays;
+	const int r0 = H / 4, r1 = H - H / 4;
+	RayRange mid = {r0 * W, r1 * W, 0, 0};
+	RayRange out = {0, r0 * W, r1 * W, H * W};
+	const unsigned n_mid = (unsigned)((r1 - r0) * W), n_out = (unsigned)(c->n_rays - n_mid);
+	Fallback *fb_mid = c->d_fb, *fb_out = c->d_fb + n_mid;
+	hipStream_t A = c->stream, B = c->stream2;
+
+	HCK(c, hipMemsetAsync(c->d_ctr, 0, 2 * sizeof(Counters), A));
+	HCK(c, hipEventRecord(c->ev[0], A));
+	HCK(c, hipEventRecord(c->ev[2], A));
+	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, mid, c->d_ctr,
+	                       fb_mid, (unsigned)c->pool_blocks, A));
+	HCK(c, hipEventRecord(c->ev[3], A));
+	HCK(c, hipStreamWaitEvent(B, c->ev[3], 0));
+	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb_mid,
+	                           c->d_ctr, n_mid, B));
+	HCK(c, hipEventRecord(c->ev[6], B));
+	HCK(c, hipEventRecord(c->ev[4], A));
+	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, out,
+	                       c->d_ctr + 1, fb_out, (unsigned)c->pool_blocks, A));
+	HCK(c, hipEventRecord(c->ev[5], A));
+	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb_out,
+	                           c->d_ctr + 1, n_out, A));
+	HCK(c, hipStreamWaitEvent(A, c->ev[6], 0));
+	HCK(c, hipEventRecord(c->ev[1], A));
+	Counters h[2];
+	HCK(c, hipMemcpyAsync(h, c->d_ctr, 2 * sizeof(Counters), hipMemcpyDeviceToHost, A));
+	HCK(c, hipStreamSynchronize(A));
+	if (h[0].n_fb > n_mid || h[1].n_fb > n_out)
+		return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow");
+	float ms = 0.f, k0 = 0.f, k1 = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-	HCK(c, hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
-	c->kern_ns = 1e6 * (double)kms;
-	c->kern_launches = 1;
-	c->last_fallback = nfb;
+	HCK(c, hipEventElapsedTime(&k0, c->ev[2], c->ev[3]));
+	HCK(c, hipEventElapsedTime(&k1, c->ev[4], c->ev[5]));
+	c->kern_ns = 1e6 * ((double)k0 + (double)k1);
+	c->kern_launches = (n_mid > 0) + (n_out > 0);
+	c->last_fallback = (int64_t)h[0].n_fb + h[1].n_fb;
+	c->last_ctr = h[0];
+	c->last_ctr.n_fb += h[1].n_fb;
+	c->last_ctr.pool_steps += h[1].pool_steps;
+	c->last_ctr.fb_steps += h[1].fb_steps;
 	if (ns)
 		*ns = 1e6 * (double)ms;
 	return GRAYHIP_OK;

@@ -39,6 +39,12 @@ struct Counters {
 	unsigned long long fb_steps;       /* RK4 steps taken in FP64 fallback   */
 };
 
+/* Rays a pool pass covers: queue index i maps to b0 + i for i < e0 - b0,
+ * then to b1 + (i - (e0 - b0)) for the rest, up to e1. */
+struct RayRange {
+	int b0, e0, b1, e1;
+};
+
 /* A ray handed from the FP32 pass to FP64 re-integration. */
 struct Fallback {
 	int g;           /* ray index                                        */

@@ -30,13 +30,16 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
                                                   double *__restrict__ data,
                                                   double *__restrict__ info,
                                                   int64_t *__restrict__ steps,
-                                                  size_t n_rays, Counters *__restrict__ ctr,
+                                                  size_t n_rays, RayRange rr,
+                                                  Counters *__restrict__ ctr,
                                                   Fallback *__restrict__ fb)
 {
+	const unsigned n0 = (unsigned)(rr.e0 - rr.b0), n_all = n0 + (unsigned)(rr.e1 - rr.b1);
 	constexpr int ND = 8 + 2 * NF;
 	const unsigned lane = __lane_id();
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
+	const RtConst<NF> rc = make_rtconst<NF>(p);
 
 	float hi[ND], lo[ND];
 	long g = -1;
@@ -56,8 +59,8 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			base = __shfl(base, leader);
 			if (need) {
 				unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-				if ((size_t)idx < n_rays) {
-					g = idx;
+				if (idx < n_all) {
+					g = idx < n0 ? (long)rr.b0 + idx : (long)rr.b1 + (idx - n0);
 #pragma unroll
 					for (int i = 0; i < ND; ++i) {
 						double v = data[(size_t)i * n_rays + g];
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 				g = -1;
 				continue;
 			}
-			rk4_f32c<NF>(p, hi, lo, (float)hd);
+			rk4_f32c<NF>(p, rc, hi, lo, (float)hd);
 			++st;
 			float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 			if (!__builtin_isfinite(chk)) {
@@ -151,13 +154,13 @@ hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks)
 }
 
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
-                           int64_t *steps, size_t n_rays, Counters *ctr, Fallback *fb,
-                           unsigned n_blocks, hipStream_t st)
+                           int64_t *steps, size_t n_rays, const RayRange &rr, Counters *ctr,
+                           Fallback *fb, unsigned n_blocks, hipStream_t st)
 {
-	if (n_rays == 0 || n_blocks == 0)
+	if (rr.e0 - rr.b0 + rr.e1 - rr.b1 <= 0 || n_blocks == 0)
 		return hipSuccess;
 #define L(NF) hipLaunchKernelGGL(k_pool_f32<NF>, dim3(n_blocks), dim3(256), 0, st, \
-	                         p, sr, data, info, steps, n_rays, ctr, fb)
+	                         p, sr, data, info, steps, n_rays, rr, ctr, fb)
 	GRAY_NF_SWITCH(p.n_freq, L)
 #undef L
 	return hipGetLastError();

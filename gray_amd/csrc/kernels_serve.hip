@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 	const int s = list[idx];
 	const SrvRequest &R = reqs[S.req[s]];
 	const LaneKerr P = lane_params(R.p);
+	const RtConst<SRV_NF> rc = make_rtconst<SRV_NF>(P);
 	const StepRule sr = R.sr;
 	float hi[SRV_NC], lo[SRV_NC];
 #pragma unroll
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 			state = 2;
 			break;
 		}
-		rk4_f32c<SRV_NF>(P, hi, lo, (float)hd);
+		rk4_f32c<SRV_NF>(P, rc, hi, lo, (float)hd);
 		++st;
 		float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 		if (!__builtin_isfinite(chk)) {

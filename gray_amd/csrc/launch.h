@@ -28,8 +28,8 @@ hipError_t launch_info_f64(const RayParams &p, const double *data, double *info,
 
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
-                           int64_t *steps, size_t n_rays, Counters *ctr, Fallback *fb,
-                           unsigned n_blocks, hipStream_t st);
+                           int64_t *steps, size_t n_rays, const RayRange &rr, Counters *ctr,
+                           Fallback *fb, unsigned n_blocks, hipStream_t st);
 
 hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *data, double *info,
                              int64_t *steps, size_t n_rays, hipStream_t st);

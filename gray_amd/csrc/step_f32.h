@@ -9,46 +9,122 @@
 
 namespace gray {
 
+/* Per-channel constants of the synthetic emissivity, nu_i^-1/2 and
+ * 0.1 nu_i^-5/2, so that a transfer RHS needs one rsqrt for the redshift
+ * and one exp per channel. */
+template <int NF> struct RtConst {
+	float c[NF > 0 ? NF : 1], d[NF > 0 ? NF : 1];
+};
+
 template <int NF, class P>
-__device__ __forceinline__ void rhs_f32(const P &p, const float *s, float *ds)
+__device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
+{
+	RtConst<NF> r;
+#pragma unroll
+	for (int i = 0; i < (NF > 0 ? NF : 1); ++i) {
+		const double nu = NF > 0 ? (double)p.nu[i] : 1.0;
+		r.c[i] = (float)(1.0 / sqrt(nu));
+		r.d[i] = (float)(0.1 / (nu * nu * sqrt(nu)));
+	}
+	return r;
+}
+
+/* The transfer RHS of sim-org/rt.cl:185-198 with the synthetic disk,
+ * evaluated from the Kerr-Schild quantities gr_rhs already has:
+ *   k_mu = eta u + f (l.u) l_mu,   g(v,v) = eta(v,v) + f (l.v)^2
+ * for the Keplerian v = (1, -y Omega, x Omega, 0); then
+ *   dI_i = -n nu_i^-1/2 s^-1/2 exp(-tau_i) / (s^2 + 1e-28)
+ *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2       (s = -k.u_fluid). */
+template <int NF, class P>
+__device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, const V4<float> &q,
+                                           const V4<float> &u, const KsAux<float> &K,
+                                           const float *tau, float *dI, float *dtau)
+{
+	typedef Mth<float> M;
+	float n = 0.f, sh = 1.f;
+	if (p.fluid == GRAYHIP_FLUID_DISK) {
+		const float R2 = q.x * q.x + q.y * q.y;
+		if (K.r >= (float)p.r_isco && R2 > 0.f) {
+			const float rs = K.r * M::sqrt_(K.r);
+			const float Om = M::div(1.f, rs + (float)p.a);
+			const float lv = 1.f + Om * (q.x * K.ly - q.y * K.lx);
+			const float vv = (Om * Om * R2 - 1.f) + K.f * lv * lv;
+			if (vv < 0.f) {
+				const float ut = __builtin_amdgcn_rsqf(-vv);
+				const float flu = K.f * K.lu;
+				const float kx = u.x + flu * K.lx, ky = u.y + flu * K.ly;
+				const float kv = (flu - u.t) + Om * (q.x * ky - q.y * kx);
+				const float s_ = -kv * ut;
+				if (s_ > 0.f) {
+					sh = s_;
+					n = M::div(1.f, rs) * M::exp_(M::div(-(q.z * q.z), 0.18f * R2));
+				}
+			}
+		}
+	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
+		n = 1.f;
+	}
+	if (n == 0.f) {
+#pragma unroll
+		for (int i = 0; i < NF; ++i) {
+			dI[i] = 0.f;
+			dtau[i] = 0.f;
+		}
+		return;
+	}
+	const float rsh = __builtin_amdgcn_rsqf(sh);
+	const float A = n * rsh * M::div(1.f, sh * sh + 1e-28f);
+	const float Bc = n * (rsh * rsh) * rsh;
+#pragma unroll
+	for (int i = 0; i < NF; ++i) {
+		dI[i] = -(A * rc.c[i]) * M::exp_(-tau[i]);
+		dtau[i] = -(Bc * rc.d[i]);
+	}
+}
+
+template <int NF, class P>
+__device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const float *s,
+                                        float *ds)
 {
 	V4<float> q = {s[0], s[1], s[2], s[3]};
 	V4<float> u = {s[4], s[5], s[6], s[7]};
-	V4<float> a = gr_rhs((float)p.a, q, u);
+	KsAux<float> K;
+	V4<float> a = gr_rhs((float)p.a, q, u, NF > 0 ? &K : nullptr);
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if (NF > 0)
-		rt_rhs<float, NF>(p, q, u, s + 8 + NF, ds + 8, ds + 8 + NF);
+		rt_rhs_f32<NF, P>(p, rc, q, u, K, s + 8 + NF, ds + 8, ds + 8 + NF);
 }
 
 /* Classical RK4 (sim-org/RK4.cl:37-47) with stages taken from hi and the
  * increment added to (hi, lo) by compensated summation. */
 template <int NF, class P>
-__device__ __forceinline__ void rk4_f32c(const P &p, float *hi, float *lo, float dt)
+__device__ __forceinline__ void rk4_f32c(const P &p, const RtConst<NF> &rc, float *hi, float *lo,
+                                         float dt)
 {
 	constexpr int ND = 8 + 2 * NF;
 	float k[ND], acc[ND], x[ND];
 	const float hdt = 0.5f * dt;
 
-	rhs_f32<NF, P>(p, hi, k);
+	rhs_f32<NF, P>(p, rc, hi, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] = k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	rhs_f32<NF, P>(p, x, k);
+	rhs_f32<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	rhs_f32<NF, P>(p, x, k);
+	rhs_f32<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + dt * k[i];
 	}
-	rhs_f32<NF, P>(p, x, k);
+	rhs_f32<NF, P>(p, rc, x, k);
 	const float dt6 = dt * (1.0f / 6.0f);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
