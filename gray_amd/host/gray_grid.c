@@ -4,13 +4,13 @@
  * sim-org/gray.c:30-59 and :147-306, over a resident device spacetime
  * (grayhip_spacetime) and a gridded render (grayhip_ctx).
  *
- * Decisions for what the reference leaves undefined:
- *   D13 rays start at q^t = 0 while the grid time is t_init + q^t, and the
- *       time weight is clamped, so only t_init must lie inside the data
- *       (gray.c:194-204 also requires t_final, which with dt_dump < 0 and
- *       levels {0, 1} admits only frozen runs);
- *   D14 a frozen run uses the level bracketing t_init (gray.c:214 means to);
- *   D7  the coordinate map is detected from the axes (uniform or fisheye).
+ * Decisions for behaviour the reference leaves undefined:
+ *   - rays start at q^t = 0 while the grid time is t_init + q^t, and the
+ *     time weight is clamped, so only t_init must lie inside the data
+ *     (gray.c:194-204 also requires t_final, which with dt_dump < 0 and
+ *     levels {0, 1} admits only frozen runs);
+ *   - a frozen run uses the level bracketing t_init (gray.c:214 means to);
+ *   - the coordinate map is detected from the axes (uniform or fisheye).
  */
 #define _GNU_SOURCE
 #include "gray_grid.h"

@@ -5,7 +5,7 @@
  * HDF5 C library (1.10).
  *
  * Differences from the reference reader, each a decision for behaviour it
- * leaves undefined (SURVEY.md section 8.0):
+ * leaves undefined:
  *   - datasets are read in their native precision and converted, so FP32
  *     and FP64 files both load (io.c:434-435 always makes FP32 images);
  *   - axis values are read in full and the coordinate map (uniform or the

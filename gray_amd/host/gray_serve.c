@@ -90,14 +90,14 @@ static void fill_request(const gray_serve_item *it, grayhip_spacetime *const gri
 	q->n_sub = 1024;
 	q->dt = -32.0;
 	if (it->grid >= 0) {
-		/* config-3 camera inside a 80 M box */
+		/* camera at r = 35 M inside the 80 M box, 4 dumps of -32 */
 		q->spacetime = grids[it->grid];
 		q->w_img = q->h_img = 32.0;
 		q->r_obs = 35.0;
 		q->n_dump = 4;
 		q->t_init = 1.0;
 	} else {
-		/* config-2 camera and disk */
+		/* camera at r = 1000 M, synthetic disk, 128 dumps of -32 */
 		q->a_spin = it->a_spin;
 		q->w_img = q->h_img = 64.0;
 		q->r_obs = 1000.0;

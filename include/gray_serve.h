@@ -2,7 +2,7 @@
  * gray_serve.h -- the C request loop over the serving pool (libgray.so).
  *
  * Generates a seeded stream of render requests of uneven sizes (the
- * config-5 mix: resolutions {64,128,256,512,1024}^2 with weights
+ * mix: resolutions {64,128,256,512,1024}^2 with weights
  * {.3,.3,.2,.15,.05}, spin U[0, 0.998], cos(inclination) U[0, 1], 1-4
  * frequency channels, 70% analytic Kerr and 30% on one of two resident
  * gridded spacetimes), submits each at its Poisson arrival time, steps the
@@ -22,7 +22,7 @@ extern "C" {
 
 typedef struct {
 	int      n_requests;      /* requests in the stream                        */
-	uint64_t seed;            /* stream seed (0x47524159 for config 5)          */
+	uint64_t seed;            /* stream seed (0x47524159 by default)            */
 	double   rate;            /* mean arrivals per second (Poisson)             */
 	double   grid_fraction;   /* share of gridded requests (0.3)                */
 	int      max_side;        /* cap on the image side (<= 1024), for samples   */

@@ -255,7 +255,7 @@ double og_getdt(const og_params *p, const double q[4], const double u[4],
 	return dt;
 }
 
-/* Synthetic accretion disk (config-2 fluid): n = r^-1.5 exp(-z^2 /
+/* Synthetic accretion disk: n = r^-1.5 exp(-z^2 /
  * (2 (0.3 rho)^2)), zero inside r_isco, on axis, or where the Keplerian
  * velocity is not timelike.  Keplerian Omega = 1/(r^1.5 + a) about the z
  * axis; in Cartesian Kerr-Schild coordinates v = (1, -y Omega, x Omega, 0)

@@ -1,5 +1,5 @@
-"""GPU parity of the numerical-spacetime path (config-3 physics at small
-sizes) against the FP64 grid oracle, through the C ABI.
+"""GPU parity of the numerical-spacetime path (boosted Kerr-Schild
+grids at small sizes) against the FP64 grid oracle, through the C ABI.
 
 Bar: the plain FP64 form on uniform grids evaluates the oracle's sequence
 and must match it bit for bit (state and step counts), for FP64 and FP32

@@ -47,7 +47,7 @@ def test_icond_bitwise():
     assert np.array_equal(i0, i1)
 
 
-def test_config1_plain_fp64_bitwise():
+def test_kerr_geodesic_plain_fp64_bitwise():
     """64x64, a=0.9375, i=60, r_obs=1000, FP64, RT off, 128 launches."""
     d0, i0, s0 = oracle_render(A, 64, 64, 64.0, 64.0, 1000.0, 60.0, 0.0)
     d1, i1, s1, _ = gpu_render(A, 64, 64, 64.0, 64.0, 1000.0, 60.0, 0.0)
@@ -58,7 +58,7 @@ def test_config1_plain_fp64_bitwise():
     assert np.array_equal(i0, i1)
 
 
-def test_config1_reference_escape_rule_bitwise():
+def test_kerr_reference_escape_rule_bitwise():
     """The escape test of phys.cl:57 as written (never fires) and the
     reference's eps < 0.01 horizon stop; a few launches."""
     kw = dict(escape=G.ESC_REF)
@@ -69,9 +69,9 @@ def test_config1_reference_escape_rule_bitwise():
     assert np.array_equal(d0, d1)
 
 
-def test_config2_rt_plain_fp64():
-    """Disk RT with 4 channels on a 48x48 image (config 2 physics at a size
-    the oracle finishes in seconds)."""
+def test_disk_rt_plain_fp64():
+    """Disk RT with 4 channels on a 48x48 image (the a=0.9375, i=60 disk
+    workload at a size the oracle finishes in seconds)."""
     d0, i0, s0 = oracle_render(A, 48, 48, 64.0, 64.0, 1000.0, 60.0, 0.0, nu=NUS)
     d1, i1, s1, _ = gpu_render(A, 48, 48, 64.0, 64.0, 1000.0, 60.0, 0.0, nu=NUS)
     assert np.array_equal(s0, s1)

@@ -1,4 +1,4 @@
-"""Precision form (config 4 physics: a = 0.998, inclination 85 deg, disk
+"""Precision form (a = 0.998, inclination 85 deg, disk
 transfer with 4 channels) on the GPU: the FP32 form (FP32 bulk, compensated
 state and transfer sums, FP64 initial conditions and FP64 re-integration of
 collapsed-step rays) against the plain FP64 form, which the Kerr tests pin
