@@ -187,21 +187,20 @@ def grid_section(G, out):
     out["grid"] = res
 
 
-def serving_section(G, args, value, out):
+def serving_section(G, args, capacity, cap_name, out):
     from gray_amd import serve as SV
-    # Mean work of the stream's requests: 70% Kerr at ~6.4e4 steps/ray,
-    # 30% gridded at ~8e3; the rate is set to 1.2x what the FP32 form can
-    # sustain, so the pool stays saturated.
+    # Mean work of the sample's requests: analytic Kerr at ~6.4e4 steps/ray,
+    # gridded at ~1e4 steps/ray; arrivals come at 1.2x the rate the given
+    # capacity (ray-steps/s) sustains.
     o = SV.defaults(n_requests=args.serve_n, max_side=args.serve_max_side)
     items = SV.stream(o)
-    mean_rays = np.mean([i.side * i.side for i in items])
-    est_steps = mean_rays * (0.7 * 6.4e4 + 0.3 * 8e3)
-    o.rate = 1.2 * value / est_steps
+    est_steps = np.mean([i.side * i.side * (1e4 if i.grid >= 0 else 6.4e4) for i in items])
+    o.rate = 1.2 * capacity / est_steps
     rep, lat = SV.run(o)
     res = {
         "sample": f"first {args.serve_n} requests of the seeded stream (seed 0x47524159, "
                   f"image side capped at {args.serve_max_side}), Poisson rate "
-                  f"{o.rate:.3f}/s = 1.2x estimated FP32-form capacity",
+                  f"{o.rate:.3f}/s = 1.2x the measured {cap_name} capacity",
         "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"], "max_ms": rep["max_ms"],
         "wall_s": rep["wall_s"], "rays": rep["rays"], "ray_steps": rep["ray_steps"],
         "epochs": rep["epochs"], "value": rep["ray_steps_per_s"], "unit": "ray-steps/s",
@@ -247,6 +246,17 @@ def precision_section(G, out):
         "null_drift_fp32": float(np.abs(i32[esc]).max()), "null_drift_fp64": float(np.abs(i64[esc]).max()),
         "fallback_rays": nfb,
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(stride, n_threads):
@@ -324,7 +334,10 @@ def main():
         grid_section(G, out)
         log(f"grid: fp32 {out['grid']['fp32_grid']['value']:.3e} ray-steps/s")
     if not args.no_serve:
-        serving_section(G, args, value, out)
+        if "fp64" in out:
+            serving_section(G, args, out["fp64"]["value"], "plain-FP64", out)
+        else:
+            serving_section(G, args, value, "FP32-form", out)
         log(f"serving: p50 {out['serving']['p50_ms']:.0f} ms, p99 {out['serving']['p99_ms']:.0f} ms")
     if args.precision_form:
         precision_section(G, out)
@@ -334,9 +347,10 @@ def main():
         rate, steps, dt, n = cpu_baseline(args.cpu_stride, n_thr)
         out["cpu_baseline"] = {
             "value": rate, "unit": "ray-steps/s", "cores": n_thr, "kind": "port",
-            "sample": f"FP64 oracle (CPU restatement of the reference path) on every "
-                      f"{args.cpu_stride}th row and column ({n} rays, {steps} ray-steps, "
-                      f"{dt:.1f} s)",
+            "sample": f"FP64 oracle (CPU restatement of the reference FP64 path, C, "
+                      f"-O3 -ffp-contract=off, OpenMP over rays) on every {args.cpu_stride}th "
+                      f"row and column ({n} rays, {steps} ray-steps, {dt:.1f} s) on "
+                      f"{n_thr} threads of {cpu_model()}",
         }
     print(json.dumps(out), flush=True)
 

@@ -124,13 +124,24 @@ __device__ __forceinline__ void make_stencil(const GridView<S> &G, T qt, T qx, T
 	}
 }
 
-/* Box-exit stop and the fixed 0.01 cap of dyst (getrr = geteps = 1). */
+/* Coordinate speed |dx/dt|^2 above which a ray counts as captured: the
+ * horizon proxy of the gridded path.  Outside a horizon light moves at
+ * |dx/dt| of order 1; deep inside, near the singularity, the interpolated
+ * Christoffels make the speed grow until RK4 overflows. */
+#define GRID_U_CAP2 16.0
+
+/* Box-exit stop, the horizon proxy and the fixed 0.01 cap of dyst
+ * (getrr = geteps = 1); u^t = 1 in this coordinate-time form. */
 template <class T, class S>
-__device__ __forceinline__ T grid_getdt(const GridView<S> &G, T qx, T qy, T qz, T dt)
+__device__ __forceinline__ T grid_getdt(const GridView<S> &G, T qx, T qy, T qz, T ux, T uy,
+                                        T uz, T dt)
 {
 	if (!G.ref_rule) {
 		if (!(qx >= T(G.lo[0]) && qx <= T(G.hi[0])) || !(qy >= T(G.lo[1]) && qy <= T(G.hi[1])) ||
 		    !(qz >= T(G.lo[2]) && qz <= T(G.hi[2])))
+			return T(0);
+		const T v2 = (ux * ux + uy * uy) + uz * uz;
+		if (!(v2 <= T(GRID_U_CAP2)))
 			return T(0);
 	}
 	const T cap = (T(0.01) * T(1)) * T(1);

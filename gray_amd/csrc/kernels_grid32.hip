@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_grid_evolve_f32(GridView<S> G, StepRule
 	int64_t nst = 0;
 	int dh;
 	for (int h = 0; h < sr.n; h += dh) {
-		float gdt = grid_getdt<float, S>(G, hi[1], hi[2], hi[3], dts);
+		float gdt = grid_getdt<float, S>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
 		double xd = (double)gdt / sr.dt * (double)sr.n;
 		dh = (xd == xd) ? (int)xd : 0;
 		if (!dh)

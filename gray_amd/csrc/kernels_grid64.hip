@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_grid_evolve_f64(GridView<S> G, StepRule
 	int64_t nst = 0;
 	int dh;
 	for (int h = 0; h < sr.n; h += dh) {
-		double x = grid_getdt<double, S>(G, s[1], s[2], s[3], sr.dts) / sr.dt * (double)sr.n;
+		double x = grid_getdt<double, S>(G, s[1], s[2], s[3], s[5], s[6], s[7], sr.dts) / sr.dt * (double)sr.n;
 		dh = (x == x) ? (int)x : 0;
 		if (!dh)
 			break;

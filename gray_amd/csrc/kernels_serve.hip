@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_srv_grid32(const SrvRequest *__restrict
 	const double dt_over_n = sr.dt / (double)sr.n;
 	bool fin = false;
 	for (int k = 0; k < K; ++k) {
-		float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], dts);
+		float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
 		double xd = (double)gdt / sr.dt * (double)sr.n;
 		int dh = (xd == xd) ? (int)xd : 0;
 		if (!dh) {

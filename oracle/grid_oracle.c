@@ -198,13 +198,22 @@ void og_grid_icond(const og_grid *G, double r_obs, double si, double ci, double 
 }
 
 /* phys.cl:48-64 with dyst's getrr = geteps = 1 (dyst.cl:43-53): the cap is
- * 0.01; the ray stops outside the grid box (unless ref_rule). */
-double og_grid_getdt(const og_grid *G, const double q[4], double dt)
+ * 0.01.  Unless ref_rule, the ray stops outside the grid box, and also where
+ * its coordinate speed |dx/dt|^2 = u_x^2 + u_y^2 + u_z^2 (u^t = 1 in this
+ * coordinate-time form) exceeds OG_GRID_U_CAP2: the horizon proxy.  Outside
+ * a horizon light moves at |dx/dt| of order 1; deep inside, near the
+ * singularity, the interpolated Christoffels make the speed grow without
+ * bound until RK4 overflows. */
+double og_grid_getdt(const og_grid *G, const double q[4], const double u[4], double dt)
 {
-	if (!G->ref_rule)
+	if (!G->ref_rule) {
 		for (int a = 0; a < 3; ++a)
 			if (!(q[a + 1] >= G->lo[a] && q[a + 1] <= G->hi[a]))
 				return 0.0;
+		const double v2 = (u[1] * u[1] + u[2] * u[2]) + u[3] * u[3];
+		if (!(v2 <= OG_GRID_U_CAP2))
+			return 0.0;
+	}
 	const double cap = (0.01 * 1.0) * 1.0;
 	if (fabs(dt) > cap)
 		return dt > 0.0 ? cap : -cap;
@@ -238,7 +247,7 @@ int64_t og_grid_evolve_ray(const og_grid *G, double s[8], double dt, size_t n_su
 	int h, dh;
 	*done = 0;
 	for (h = 0; h < n; h += dh) {
-		double x = og_grid_getdt(G, s, dts) / dt * (double)n;
+		double x = og_grid_getdt(G, s, s + 4, dts) / dt * (double)n;
 		dh = (x == x) ? (int)x : 0;
 		if (!dh) {
 			*done = 1;

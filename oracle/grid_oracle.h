@@ -21,6 +21,9 @@ extern "C" {
 
 enum { OG_MAP_UNIFORM = 0, OG_MAP_FISHEYE = 1 };
 
+/* Horizon proxy of the box-exit rule: stop where |dx/dt|^2 > 16. */
+#define OG_GRID_U_CAP2 16.0
+
 typedef struct {
 	int    n[3];          /* nx, ny, nz                                   */
 	double lo[3], hi[3];  /* first and last axis values                   */
@@ -38,7 +41,7 @@ double  og_grid_getuu(const og_grid *G, const double q[4], const double u[4]);
 void    og_grid_rhs(const og_grid *G, const double q[4], const double u[4], double du[4]);
 void    og_grid_icond(const og_grid *G, double r_obs, double si, double ci, double sj,
                       double cj, double alpha, double beta, double q[4], double u[4]);
-double  og_grid_getdt(const og_grid *G, const double q[4], double dt);
+double  og_grid_getdt(const og_grid *G, const double q[4], const double u[4], double dt);
 void    og_grid_integrate(const og_grid *G, double s[8], double dt);
 int64_t og_grid_evolve_ray(const og_grid *G, double s[8], double dt, size_t n_sub, int *done);
 void    og_grid_icond_all(const og_grid *G, size_t w_rays, size_t h_rays, double w_img,

@@ -246,7 +246,7 @@ def _glib():
         L.og_grid_getuu.restype = C.c_double
         L.og_grid_rhs.argtypes = [gp, dp, dp, dp]
         L.og_grid_icond.argtypes = [gp] + [C.c_double] * 7 + [dp, dp]
-        L.og_grid_getdt.argtypes = [gp, dp, C.c_double]
+        L.og_grid_getdt.argtypes = [gp, dp, dp, C.c_double]
         L.og_grid_getdt.restype = C.c_double
         L.og_grid_integrate.argtypes = [gp, dp, C.c_double]
         L.og_grid_evolve_ray.argtypes = [gp, dp, C.c_double, C.c_size_t, C.POINTER(C.c_int)]
@@ -298,8 +298,8 @@ class GridSpacetime:
     def getuu(self, q, u):
         return _glib().og_grid_getuu(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)))
 
-    def getdt(self, q, dt):
-        return _glib().og_grid_getdt(C.byref(self.g), _dp(_vec(q)), dt)
+    def getdt(self, q, dt, u=(1.0, 0.0, 0.0, 0.0)):
+        return _glib().og_grid_getdt(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)), dt)
 
     def icond(self, r_obs, i_obs, j_obs, alpha, beta):
         si, ci, sj, cj = camera_trig(i_obs, j_obs)

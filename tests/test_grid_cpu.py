@@ -176,6 +176,45 @@ def test_grid_icond_null_step_rule_and_box_exit():
     assert G.getdt([0.0, 40.5, 0.0, 0.0], -0.03125) == 0.0
     Gref = O.GridSpacetime((33,) * 3, (-40,) * 3, (40,) * 3, [gam], [met], ref_rule=True)
     assert Gref.getdt([0.0, 40.5, 0.0, 0.0], -0.03125) == -0.01
+    # horizon proxy: coordinate speed |dx/dt| above 4 stops the ray; NaN too
+    assert G.getdt(q, -0.03125, u=[1.0, 0.0, 4.0, 0.0]) == -0.01
+    assert G.getdt(q, -0.03125, u=[1.0, 0.0, 4.0, 1e-7]) == 0.0
+    assert G.getdt(q, -0.03125, u=[1.0, np.nan, 0.0, 0.0]) == 0.0
+    assert Gref.getdt(q, -0.03125, u=[1.0, 0.0, 40.0, 0.0]) == -0.01
+
+
+def test_grid_horizon_proxy_stops_only_captured_rays():
+    """Ten pixels of the 512^2, 32 M x 32 M, r_obs = 35, i = 45 image of the
+    128^3 boosted Kerr-Schild grid (a = 0.7, v = 0.3 along x, levels 0 and
+    1) whose rays wind into the hole.  Under the reference rule they run
+    into the singularity and overflow to NaN; under the default rule the
+    speed cap stops each of them early, finite, inside the horizon
+    (|x| < 1.85 for a = 0.7)."""
+    n = 128
+    xs = S.axis(-40, 40, n)
+    b = S.bks(1.0, 0.7, 0.3, (1, 0, 0))
+    lv = [S.bks_sample(b, t, xs, xs, xs) for t in (0.0, 1.0)]
+    pix = [(288, 220), (292, 220), (220, 224), (292, 224), (296, 224), (220, 228),
+           (292, 228), (296, 228), (284, 232), (288, 232)]
+    out = []
+    for ref in (False, True):
+        G = O.GridSpacetime((n,) * 3, (-40.0,) * 3, (40.0,) * 3, [lv[0][0], lv[1][0]],
+                            [lv[0][1], lv[1][1]], times=(0.0, 1.0), t_off=1.0, ref_rule=ref)
+        rows = []
+        for gi, gj in pix:
+            q, u = G.icond(35.0, 45.0, 0.0, ((gi + 0.5) / 512 - 0.5) * 32.0,
+                           ((gj + 0.5) / 512 - 0.5) * 32.0)
+            rows.append(np.concatenate([q, u]))
+        d = np.array(rows)
+        steps = np.zeros(len(d), dtype=np.int64)
+        for _ in range(4):
+            G.evolve_all(d, np.zeros(len(d)), -32.0, 1024, steps, 8)
+        out.append((d, steps))
+    (d, steps), (dref, _) = out
+    assert not np.isfinite(dref).all(axis=1).any()
+    assert np.isfinite(d).all()
+    assert (steps < 4 * 3201).all()
+    assert (np.linalg.norm(d[:, 1:4], axis=1) < 1.85).all()
 
 
 def test_grid_ray_leaves_the_box():
