@@ -13,8 +13,11 @@
  *     [0, 1] (interp.cl:204 extrapolates);
  *   - the initial direction is null-normalised with the interpolated metric
  *     (dyst.cl:89-110 does not normalise) unless raw_icond is set;
- *   - a ray stops when it is outside the grid box (the reference's dyst
- *     getrr/geteps stubs, dyst.cl:43-53, never stop it).
+ *   - a ray stops when it is outside the grid box or its coordinate speed
+ *     marks it as captured (the reference's dyst getrr/geteps stubs,
+ *     dyst.cl:43-53, never stop it);
+ *   - radiative transfer uses the reference flow (flow.cl:36-58) and
+ *     emissivities (rt.cl, restated in rt_oracle.c).
  * Evaluation order is written out so the FP64 GPU kernel can match it.
  */
 #include "grid_oracle.h"
@@ -148,11 +151,10 @@ double og_grid_getuu(const og_grid *G, const double q[4], const double u[4])
 }
 
 /* dyst.cl:112-194: a^mu = -u.Gamma^mu.u + (u.Gamma^t.u) u^mu. */
-void og_grid_rhs(const og_grid *G, const double q[4], const double u[4], double du[4])
+static void rhs_at(const og_grid *G, const stencil *Sp, const double u[4], double du[4])
 {
-	stencil S;
+	const stencil S = *Sp;
 	double gam[40], uGu[4];
-	make_stencil(G, q, &S);
 	for (int c = 0; c < 40; ++c)
 		gam[c] = sample(G, G->gam, 40, c, &S);
 	for (int mu = 0; mu < 4; ++mu) {
@@ -165,6 +167,51 @@ void og_grid_rhs(const og_grid *G, const double q[4], const double u[4], double 
 	}
 	for (int mu = 0; mu < 4; ++mu)
 		du[mu] = -uGu[mu] + uGu[0] * u[mu];
+}
+
+void og_grid_rhs(const og_grid *G, const double q[4], const double u[4], double du[4])
+{
+	stencil S;
+	make_stencil(G, q, &S);
+	rhs_at(G, &S, u, du);
+}
+
+/* flow.cl:36-58: n_e = rho interpolated like every other field; the fluid
+ * is at rest, u_fluid = (1,0,0,0), so shift = -k.u_fluid = -k_t with
+ * k_t = g_{t nu} u^nu (the t row of dyst.cl:55-81). */
+static void flow_at(const og_grid *G, const stencil *S, const double u[4], double *ne,
+                    double *shift)
+{
+	double m[4];
+	for (int c = 0; c < 4; ++c)
+		m[c] = sample(G, G->met, 10, c, S);
+	const double kt = ((m[0] * u[0] + m[1] * u[1]) + m[2] * u[2]) + m[3] * u[3];
+	*shift = -kt;
+	*ne = sample(G, G->rho, 1, 0, S);
+}
+
+void og_grid_flow(const og_grid *G, const double q[4], const double u[4], double *ne,
+                  double *shift)
+{
+	stencil S;
+	make_stencil(G, q, &S);
+	flow_at(G, &S, u, ne, shift);
+}
+
+/* phys.cl:66-74 on a gridded spacetime: the geodesic part, then rt_rhs of
+ * rt.cl:185-198 with the flow above (T_e = 1e12, b = 0, bkcos = 0). */
+void og_grid_rhs_all(const og_grid *G, const double *s, double *ds)
+{
+	stencil S;
+	make_stencil(G, s, &S);
+	ds[0] = s[4]; ds[1] = s[5]; ds[2] = s[6]; ds[3] = s[7];
+	rhs_at(G, &S, s + 4, ds + 4);
+	if (G->n_freq > 0) {
+		double ne, shift;
+		flow_at(G, &S, s + 4, &ne, &shift);
+		og_rt_thermal(G->rt, G->n_freq, G->nu, G->m_adm, ne, OG_RT_TE_FLOW, 0.0, 0.0, shift,
+		              s + 8, s + 8 + G->n_freq, ds + 8, ds + 8 + G->n_freq);
+	}
 }
 
 /* dyst.cl:89-110 camera, plus (unless raw_icond) the null normalisation of
@@ -220,26 +267,24 @@ double og_grid_getdt(const og_grid *G, const double q[4], const double u[4], dou
 	return dt;
 }
 
-/* RK4.cl:37-47 on the 8 geodesic components. */
-void og_grid_integrate(const og_grid *G, double s[8], double dt)
+/* RK4.cl:37-47 on the whole state (geodesic and transfer components). */
+void og_grid_integrate(const og_grid *G, double *s, double dt)
 {
-	double k1[8], k2[8], k3[8], k4[8], x[8];
+	const int nd = og_grid_n_data(G);
+	double k1[24], k2[24], k3[24], k4[24], x[24];
 	const double hdt = 0.5 * dt;
-#define RHS(in, out) do { out[0] = in[4]; out[1] = in[5]; out[2] = in[6]; out[3] = in[7]; \
-		og_grid_rhs(G, in, in + 4, out + 4); } while (0)
-	RHS(s, k1);
-	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k1[i];
-	RHS(x, k2);
-	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k2[i];
-	RHS(x, k3);
-	for (int i = 0; i < 8; ++i) x[i] = s[i] + dt * k3[i];
-	RHS(x, k4);
-#undef RHS
-	for (int i = 0; i < 8; ++i)
+	og_grid_rhs_all(G, s, k1);
+	for (int i = 0; i < nd; ++i) x[i] = s[i] + hdt * k1[i];
+	og_grid_rhs_all(G, x, k2);
+	for (int i = 0; i < nd; ++i) x[i] = s[i] + hdt * k2[i];
+	og_grid_rhs_all(G, x, k3);
+	for (int i = 0; i < nd; ++i) x[i] = s[i] + dt * k3[i];
+	og_grid_rhs_all(G, x, k4);
+	for (int i = 0; i < nd; ++i)
 		s[i] = s[i] + (dt * ((k1[i] + 2.0 * (k2[i] + k3[i])) + k4[i])) / 6.0;
 }
 
-int64_t og_grid_evolve_ray(const og_grid *G, double s[8], double dt, size_t n_sub, int *done)
+int64_t og_grid_evolve_ray(const og_grid *G, double *s, double dt, size_t n_sub, int *done)
 {
 	const int n = (int)((INT_MAX / (long)n_sub) * (long)n_sub);
 	const double dts = dt / (double)n_sub;
@@ -271,8 +316,10 @@ void og_grid_icond_all(const og_grid *G, size_t w_rays, size_t h_rays, double w_
 		size_t gi = (size_t)g % w_rays, gj = (size_t)g / w_rays;
 		double alpha = (((double)gi + 0.5) / (double)w_rays - 0.5) * w_img;
 		double beta  = (((double)gj + 0.5) / (double)h_rays - 0.5) * h_img;
-		double *s = data + (size_t)g * 8;
+		double *s = data + (size_t)g * (size_t)og_grid_n_data(G);
 		og_grid_icond(G, r_obs, si, ci, sj, cj, alpha, beta, s, s + 4);
+		for (int i = 8; i < og_grid_n_data(G); ++i)
+			s[i] = 0.0;                   /* rt_icond, rt.cl:179-183 */
 		if (info)
 			info[g] = og_grid_getuu(G, s, s + 4);
 	}
@@ -290,7 +337,7 @@ int64_t og_grid_evolve_all(const og_grid *G, size_t n_rays, double *data, double
 #endif
 #pragma omp parallel for schedule(dynamic, 8) reduction(+:total)
 	for (long g = 0; g < (long)n_rays; ++g) {
-		double *s = data + (size_t)g * 8;
+		double *s = data + (size_t)g * (size_t)og_grid_n_data(G);
 		int done;
 		int64_t st = og_grid_evolve_ray(G, s, dt, n_sub, &done);
 		total += st;
@@ -308,5 +355,7 @@ double og_grid_sample(const og_grid *G, int which, int comp, const double q[4])
 	make_stencil(G, q, &S);
 	if (which == 0)
 		return sample(G, G->gam, 40, comp, &S);
+	if (which == 2)
+		return sample(G, G->rho, 1, 0, &S);
 	return sample(G, G->met, 10, comp, &S);
 }

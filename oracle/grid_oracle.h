@@ -7,13 +7,16 @@
  * 40 Christoffels Gamma^mu_{ab} in file order (mu-major, then
  * tt tx ty tz xx xy xz yy yz zz), met[l][p*10 + c] the 10 metric
  * components g_tt g_tx g_ty g_tz g_xx g_xy g_xz g_yy g_yz g_zz, for the
- * two time levels l = 0 (t1) and 1 (t2).
+ * two time levels l = 0 (t1) and 1 (t2).  Ray states are AoS with
+ * og_grid_n_data reals per ray: q^mu, u^mu, then I_i and tau_i.
  */
 #ifndef GRID_ORACLE_H
 #define GRID_ORACLE_H
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "rt_oracle.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -34,7 +37,16 @@ typedef struct {
 	int    ref_rule;      /* 1: no box-exit stop (dyst.cl as written)      */
 	const double *gam[2];
 	const double *met[2];
+	/* radiative transfer with the reference flow (flow.cl:36-58): density
+	 * from the rho datasets, T_e = 1e12, b = 0, fluid at rest */
+	const double *rho[2];       /* per level, n_points; NULL without transfer */
+	int    n_freq;              /* channels; state = 8 + 2 n_freq reals      */
+	double nu[8];
+	double m_adm;               /* M_ADM in solar masses (param.opts:4)      */
+	const og_rt_consts *rt;
 } og_grid;
+
+static inline int og_grid_n_data(const og_grid *G) { return 8 + 2 * G->n_freq; }
 
 void    og_grid_down(const og_grid *G, const double q[4], const double u[4], double k[4]);
 double  og_grid_getuu(const og_grid *G, const double q[4], const double u[4]);
@@ -42,14 +54,20 @@ void    og_grid_rhs(const og_grid *G, const double q[4], const double u[4], doub
 void    og_grid_icond(const og_grid *G, double r_obs, double si, double ci, double sj,
                       double cj, double alpha, double beta, double q[4], double u[4]);
 double  og_grid_getdt(const og_grid *G, const double q[4], const double u[4], double dt);
-void    og_grid_integrate(const og_grid *G, double s[8], double dt);
-int64_t og_grid_evolve_ray(const og_grid *G, double s[8], double dt, size_t n_sub, int *done);
+/* Full state derivative (geodesic and, with n_freq > 0, transfer). */
+void    og_grid_rhs_all(const og_grid *G, const double *s, double *ds);
+/* The flow at (q, u): density, and shift = -k.u_fluid with u_fluid at rest. */
+void    og_grid_flow(const og_grid *G, const double q[4], const double u[4], double *ne,
+                     double *shift);
+void    og_grid_integrate(const og_grid *G, double *s, double dt);
+int64_t og_grid_evolve_ray(const og_grid *G, double *s, double dt, size_t n_sub, int *done);
 void    og_grid_icond_all(const og_grid *G, size_t w_rays, size_t h_rays, double w_img,
                           double h_img, double r_obs, double si, double ci, double sj,
                           double cj, double *data, double *info);
 int64_t og_grid_evolve_all(const og_grid *G, size_t n_rays, double *data, double *info,
                            double dt, size_t n_sub, int64_t *steps_out, int n_threads);
-/* One interpolated field: which = 0 Christoffel comp (0..39), 1 metric (0..9). */
+/* One interpolated field: which = 0 Christoffel comp (0..39), 1 metric
+ * (0..9), 2 density. */
 double  og_grid_sample(const og_grid *G, int which, int comp, const double q[4]);
 
 #ifdef __cplusplus

@@ -217,6 +217,18 @@ def evolve_subset(p, data, stride, dt, n_sub, n_dump, n_threads=0):
 MAP_UNIFORM, MAP_FISHEYE = 0, 1
 
 
+RT_T_GRID = 60
+RT_TE_FLOW = 1e12
+M_ADM_DEFAULT = 4.3e6
+
+
+class RtConsts(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "planck1", "planck2", "c", "gaunt_a", "gaunt_b", "gaunt_c", "gaunt_d", "gaunt_e",
+        "ff_x", "ff_y", "ff_f", "syn_nus", "syn_f", "syn_k", "k2_h")] + [
+        ("log_k2", C.c_double * (RT_T_GRID + 1))]
+
+
 class Grid(C.Structure):
     _fields_ = [
         ("n", C.c_int * 3),
@@ -230,7 +242,70 @@ class Grid(C.Structure):
         ("ref_rule", C.c_int),
         ("gam", C.c_void_p * 2),
         ("met", C.c_void_p * 2),
+        ("rho", C.c_void_p * 2),
+        ("n_freq", C.c_int),
+        ("nu", C.c_double * MAX_FREQ),
+        ("m_adm", C.c_double),
+        ("rt", C.POINTER(RtConsts)),
     ]
+
+
+_rt_set = False
+
+
+def _rlib():
+    global _rt_set
+    L = lib()
+    if not _rt_set:
+        rp, dp, d = C.POINTER(RtConsts), C.POINTER(C.c_double), C.c_double
+        L.og_log_k2.argtypes = [d]
+        L.og_log_k2.restype = d
+        L.og_rt_consts_init.argtypes = [rp]
+        for name, n in (("og_log_k2it", 1), ("og_b_planck", 2), ("og_gaunt", 2), ("og_j_ff", 4),
+                        ("og_j_syn", 6)):
+            fn = getattr(L, name)
+            fn.argtypes = [rp] + [d] * n
+            fn.restype = d
+        L.og_rt_thermal.argtypes = [rp, C.c_int, dp, d, d, d, d, d, d, dp, dp, dp, dp]
+        _rt_set = True
+    return L
+
+
+def log_k2(x):
+    """ln K_2(x) by quadrature (the oracle's table generator)."""
+    return _rlib().og_log_k2(x)
+
+
+class Rt:
+    """The reference's transfer physics (rt.cl) with its constants."""
+
+    def __init__(self):
+        self.c = RtConsts()
+        _rlib().og_rt_consts_init(C.byref(self.c))
+
+    def log_k2it(self, te):
+        return _rlib().og_log_k2it(C.byref(self.c), te)
+
+    def b_planck(self, nu, te):
+        return _rlib().og_b_planck(C.byref(self.c), nu, te)
+
+    def gaunt(self, x, y):
+        return _rlib().og_gaunt(C.byref(self.c), x, y)
+
+    def j_ff(self, nu, te, ne, m_adm=M_ADM_DEFAULT):
+        return _rlib().og_j_ff(C.byref(self.c), nu, te, ne, m_adm)
+
+    def j_syn(self, nu, te, ne, b, cos_theta, m_adm=M_ADM_DEFAULT):
+        return _rlib().og_j_syn(C.byref(self.c), nu, te, ne, b, cos_theta, m_adm)
+
+    def thermal(self, nu, ne, te, b, bkcos, shift, I, tau, m_adm=M_ADM_DEFAULT):
+        nu = _vec(nu)
+        n = len(nu)
+        I, tau = _vec(I), _vec(tau)
+        dI, dtau = np.zeros(n), np.zeros(n)
+        _rlib().og_rt_thermal(C.byref(self.c), n, _dp(nu), m_adm, ne, te, b, bkcos, shift,
+                              _dp(I), _dp(tau), _dp(dI), _dp(dtau))
+        return dI, dtau
 
 
 _grid_set = False
@@ -257,6 +332,8 @@ def _glib():
         L.og_grid_evolve_all.restype = C.c_int64
         L.og_grid_sample.argtypes = [gp, C.c_int, C.c_int, dp]
         L.og_grid_sample.restype = C.c_double
+        L.og_grid_rhs_all.argtypes = [gp, dp, dp]
+        L.og_grid_flow.argtypes = [gp, dp, dp, dp, dp]
         _grid_set = True
     return L
 
@@ -266,7 +343,7 @@ class GridSpacetime:
     gam[level] (n_points x 40) and met[level] (n_points x 10)."""
 
     def __init__(self, n, lo, hi, gam, met, times=(0.0, 0.0), map=MAP_UNIFORM, t_off=0.0,
-                 raw_icond=False, ref_rule=False):
+                 raw_icond=False, ref_rule=False, rho=None, nu=(), m_adm=M_ADM_DEFAULT):
         g = Grid()
         for a in range(3):
             g.n[a], g.lo[a], g.hi[a] = n[a], lo[a], hi[a]
@@ -280,7 +357,33 @@ class GridSpacetime:
             me = np.ascontiguousarray(met[min(l, len(met) - 1)], dtype=np.float64)
             self._keep += [ga, me]
             g.gam[l], g.met[l] = ga.ctypes.data, me.ctypes.data
+        if rho is not None:
+            for l in range(2):
+                r = np.ascontiguousarray(rho[min(l, len(rho) - 1)], dtype=np.float64)
+                self._keep.append(r)
+                g.rho[l] = r.ctypes.data
+        elif len(nu):
+            raise ValueError("transfer on a gridded spacetime needs the density (rho)")
+        g.n_freq = len(nu)
+        for i, v in enumerate(nu):
+            g.nu[i] = v
+        g.m_adm = m_adm
+        self.rt = Rt()
+        g.rt = C.pointer(self.rt.c)
+        self.n_data = 8 + 2 * len(nu)
         self.g = g
+
+    def rhs_all(self, s):
+        s = _vec(s)
+        ds = np.zeros(self.n_data)
+        _glib().og_grid_rhs_all(C.byref(self.g), _dp(s), _dp(ds))
+        return ds
+
+    def flow(self, q, u):
+        ne, shift = C.c_double(), C.c_double()
+        _glib().og_grid_flow(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)), C.byref(ne),
+                             C.byref(shift))
+        return ne.value, shift.value
 
     def sample(self, which, comp, q):
         return _glib().og_grid_sample(C.byref(self.g), which, comp, _dp(_vec(q)))
@@ -309,7 +412,7 @@ class GridSpacetime:
 
     def icond_all(self, w, h, w_img, h_img, r_obs, i_obs, j_obs):
         si, ci, sj, cj = camera_trig(i_obs, j_obs)
-        data, info = np.zeros((w * h, 8)), np.zeros(w * h)
+        data, info = np.zeros((w * h, self.n_data)), np.zeros(w * h)
         _glib().og_grid_icond_all(C.byref(self.g), w, h, w_img, h_img, r_obs, si, ci, sj, cj,
                                   _dp(data), _dp(info))
         return data, info
