@@ -1,7 +1,8 @@
 /*
  * grayhip_grid.hip -- resident numerical spacetimes (include/grayhip.h):
  * two device slots per spacetime, each holding one time level as
- * interleaved per-point Christoffels [p][40] and metric [p][10].  Replaces
+ * interleaved per-point Christoffels [p][40] and metric [p][10], and the
+ * density [p].  Replaces
  * the 2 x 51 OpenCL images of sim-org/gray.h:92-93 and the upload half of
  * sim-org/io.c:302-533 (the HDF5 read is host code in libgray).
  */
@@ -39,6 +40,7 @@ extern "C" void grayhip_spacetime_destroy(grayhip_spacetime *st)
 	for (int l = 0; l < 2; ++l) {
 		(void)hipFree(st->gam[l]);
 		(void)hipFree(st->met[l]);
+		(void)hipFree(st->rho[l]);
 	}
 	(void)hipFree(st->d_tmp);
 	free(st);
@@ -70,7 +72,8 @@ extern "C" int grayhip_spacetime_create(const grayhip_grid_desc *d, grayhip_spac
 	}
 	for (int l = 0; l < 2; ++l) {
 		if (hipMalloc(&s->gam[l], np * 40 * d->dtype) != hipSuccess ||
-		    hipMalloc(&s->met[l], np * 10 * d->dtype) != hipSuccess) {
+		    hipMalloc(&s->met[l], np * 10 * d->dtype) != hipSuccess ||
+		    hipMalloc(&s->rho[l], np * d->dtype) != hipSuccess) {
 			grayhip_spacetime_destroy(s);
 			return GRAYHIP_ENOMEM;
 		}
@@ -93,8 +96,24 @@ extern "C" int grayhip_spacetime_upload(grayhip_spacetime *s, int slot, double t
 		return sfail(s, GRAYHIP_EDEVICE, "upload: %s", hipGetErrorString(e));
 	s->time[slot] = time;
 	s->loaded[slot] = 1;
+	s->has_rho[slot] = 0;
 	if (slot == 1)
 		s->frozen = 0;
+	return GRAYHIP_OK;
+}
+
+extern "C" int grayhip_spacetime_upload_rho(grayhip_spacetime *s, int slot, const void *rho)
+{
+	if (!s || slot < 0 || slot > 1 || !rho)
+		return sfail(s, GRAYHIP_EINVAL, "upload_rho: bad argument");
+	if (!s->loaded[slot])
+		return sfail(s, GRAYHIP_ESTATE, "upload_rho: slot %d holds no level", slot);
+	hipError_t e;
+	if ((e = hipSetDevice(s->desc.device)) != hipSuccess ||
+	    (e = hipMemcpy(s->rho[slot], rho, s->n_points * (size_t)s->desc.dtype,
+	                   hipMemcpyHostToDevice)) != hipSuccess)
+		return sfail(s, GRAYHIP_EDEVICE, "upload_rho: %s", hipGetErrorString(e));
+	s->has_rho[slot] = 1;
 	return GRAYHIP_OK;
 }
 
@@ -103,11 +122,13 @@ extern "C" int grayhip_spacetime_shift(grayhip_spacetime *s, int to_t1)
 	if (!s)
 		return GRAYHIP_EINVAL;
 	/* swap the slots; the one that is not current any more is free */
-	void *g = s->gam[0], *m = s->met[0];
+	void *g = s->gam[0], *m = s->met[0], *r = s->rho[0];
 	double t = s->time[0];
-	int l = s->loaded[0];
-	s->gam[0] = s->gam[1]; s->met[0] = s->met[1]; s->time[0] = s->time[1]; s->loaded[0] = s->loaded[1];
-	s->gam[1] = g; s->met[1] = m; s->time[1] = t; s->loaded[1] = l;
+	int l = s->loaded[0], hr = s->has_rho[0];
+	s->gam[0] = s->gam[1]; s->met[0] = s->met[1]; s->rho[0] = s->rho[1];
+	s->time[0] = s->time[1]; s->loaded[0] = s->loaded[1]; s->has_rho[0] = s->has_rho[1];
+	s->gam[1] = g; s->met[1] = m; s->rho[1] = r; s->time[1] = t; s->loaded[1] = l;
+	s->has_rho[1] = hr;
 	(void)to_t1; /* both directions are a swap; the caller reloads the other slot */
 	return GRAYHIP_OK;
 }
@@ -124,7 +145,12 @@ extern "C" int grayhip_spacetime_set_frozen(grayhip_spacetime *s, int frozen)
 
 extern "C" size_t grayhip_spacetime_bytes(const grayhip_spacetime *s)
 {
-	return s ? 2 * s->n_points * 50 * (size_t)s->desc.dtype : 0;
+	return s ? 2 * s->n_points * 51 * (size_t)s->desc.dtype : 0;
+}
+
+int grayhip_spacetime_has_rho(const grayhip_spacetime *s)
+{
+	return s->has_rho[0] && (s->frozen || s->has_rho[1]);
 }
 
 GridDesc spacetime_view(const grayhip_spacetime *s, double t_off, int raw_icond, int ref_rule)
@@ -135,6 +161,7 @@ GridDesc spacetime_view(const grayhip_spacetime *s, double t_off, int raw_icond,
 	for (int l = 0; l < 2; ++l) {
 		g.gam[l] = s->gam[s->frozen ? 0 : l];
 		g.met[l] = s->met[s->frozen ? 0 : l];
+		g.rho[l] = s->rho[s->frozen ? 0 : l];
 	}
 	g.map = s->desc.map;
 	for (int a = 0; a < 3; ++a) {
@@ -156,11 +183,13 @@ GridDesc spacetime_view(const grayhip_spacetime *s, double t_off, int raw_icond,
 extern "C" int grayhip_spacetime_sample(grayhip_spacetime *s, int n, const double *events,
                                         int which, int comp, double t_init, double *out)
 {
-	if (!s || n < 0 || !events || !out || (which != 0 && which != 1) || comp < 0 ||
-	    comp >= (which == 0 ? 40 : 10))
+	if (!s || n < 0 || !events || !out || which < 0 || which > 2 || comp < 0 ||
+	    comp >= (which == 0 ? 40 : (which == 1 ? 10 : 1)))
 		return sfail(s, GRAYHIP_EINVAL, "sample: bad argument");
 	if (!s->loaded[0])
 		return sfail(s, GRAYHIP_ESTATE, "sample: no level loaded");
+	if (which == 2 && !grayhip_spacetime_has_rho(s))
+		return sfail(s, GRAYHIP_ESTATE, "sample: no density uploaded");
 	hipError_t e;
 	if ((e = hipSetDevice(s->desc.device)) != hipSuccess)
 		return sfail(s, GRAYHIP_EDEVICE, "sample: %s", hipGetErrorString(e));

@@ -35,6 +35,7 @@ struct grayhip_ctx {
 	bool have_icond;
 	int pool_blocks;
 	grayhip_spacetime *st;  /* numerical spacetime, or NULL (analytic Kerr) */
+	RtThermal rt;           /* emission model constants (gridded transfer)  */
 	int64_t last_fallback;
 	Counters last_ctr;
 	double kern_ns;
@@ -61,6 +62,35 @@ static int fail(grayhip_ctx *c, int code, const char *fmt, ...)
 	} while (0)
 
 static GridDesc grid_view(const grayhip_ctx *c);
+
+/* Constants of sim-org/rt.cl:28-143 in cgs (CODATA 2010 values, as there),
+ * evaluated with the same expressions as the CPU oracle. */
+void gray::rt_thermal_init(RtThermal *r, double m_adm, int n_freq, const double *nu)
+{
+	const double c_light = 2.99792458e+10, h = 6.62606957e-27, G = 6.67384800e-08;
+	const double k_b = 1.38064881e-16, ry = 2.17987197e-11, m_e = 9.10938291e-28;
+	const double m_sun = 1.98910000e+33, zeta = 1.78107241799;
+	const double pi = 3.14159265358979323846;
+	memset(r, 0, sizeof(*r));
+	r->planck1 = (2.0 * h) * c_light;
+	r->planck2 = h / (m_e * c_light);
+	r->c = c_light;
+	r->gaunt_a = sqrt(3.0 / pi);
+	r->gaunt_b = sqrt(3.0) / pi;
+	r->gaunt_c = log(4.0 / zeta);
+	r->gaunt_d = log(4.0 / pow(zeta, 2.5));
+	r->gaunt_e = sqrt(12.0);
+	r->ff_x = ((m_e * c_light) * c_light) / ry;
+	r->ff_y = h / ((m_e * c_light) * c_light);
+	r->ff_f = sqrt(((G * m_sun) / (c_light * c_light)) * 6.8e-38 /
+	               ((4.0 * pi) * sqrt(((m_e * c_light) * c_light) / k_b)));
+	r->m_adm = m_adm;
+	r->te = RT_TE_FLOW;
+	r->scale = m_adm * (r->ff_f * r->ff_f);
+	r->n_freq = n_freq;
+	for (int i = 0; i < n_freq; ++i)
+		r->nu[i] = nu[i];
+}
 
 /* Prograde ISCO (Bardeen, Press & Teukolsky 1972), M = 1. */
 static double r_isco(double a)
@@ -132,10 +162,14 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 		return fail(nullptr, GRAYHIP_EINVAL, "|a_spin| must be < 1");
 	if (cfg->escape != GRAYHIP_ESC_REF && cfg->escape != GRAYHIP_ESC_OUTWARD)
 		return fail(nullptr, GRAYHIP_EINVAL, "unknown escape rule %d", cfg->escape);
-	if (cfg->fluid < GRAYHIP_FLUID_NONE || cfg->fluid > GRAYHIP_FLUID_UNIFORM)
+	if (cfg->fluid < GRAYHIP_FLUID_NONE || cfg->fluid > GRAYHIP_FLUID_THERMAL)
 		return fail(nullptr, GRAYHIP_EINVAL, "unknown fluid model %d", cfg->fluid);
-	if (cfg->spacetime && cfg->n_freq > 0)
-		return fail(nullptr, GRAYHIP_EINVAL, "radiative transfer on a numerical spacetime is not supported");
+	if (cfg->spacetime && cfg->n_freq > 0 && cfg->fluid != GRAYHIP_FLUID_THERMAL)
+		return fail(nullptr, GRAYHIP_EINVAL, "radiative transfer on a numerical spacetime uses "
+		                                     "the thermal fluid (the reference flow)");
+	if (!cfg->spacetime && cfg->n_freq > 0 && cfg->fluid == GRAYHIP_FLUID_THERMAL)
+		return fail(nullptr, GRAYHIP_EINVAL, "the thermal fluid reads the density of a "
+		                                     "numerical spacetime");
 	if (cfg->spacetime && cfg->spacetime->desc.device != cfg->device)
 		return fail(nullptr, GRAYHIP_EINVAL, "spacetime lives on another device");
 	if (cfg->n_freq > 0 && cfg->fluid == GRAYHIP_FLUID_NONE)
@@ -173,6 +207,8 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 	p.w_rays = (int)cfg->w_rays;
 	p.h_rays = (int)cfg->h_rays;
 	c->st = cfg->spacetime;
+	rt_thermal_init(&c->rt, cfg->m_adm > 0.0 ? cfg->m_adm : 4.3e6,
+	                c->st ? cfg->n_freq : 0, cfg->nu);
 
 	int rc;
 	hipError_t e;
@@ -245,6 +281,9 @@ extern "C" int grayhip_icond(grayhip_ctx *c, double w_img, double h_img, double 
 	c->p.sj = sin(d2r * j_obs);
 	c->p.cj = cos(d2r * j_obs);
 	HCK(c, hipSetDevice(c->cfg.device));
+	if (c->st && c->rt.n_freq > 0 && !grayhip_spacetime_has_rho(c->st))
+		return fail(c, GRAYHIP_ESTATE, "transfer needs the density of every level in use "
+		                               "(grayhip_spacetime_upload_rho)");
 	if (c->st)
 		HCK(c, launch_grid_icond(grid_view(c), c->p, c->d_data, c->d_info, c->d_steps, c->n_rays,
 		                         c->stream));
@@ -265,15 +304,18 @@ static GridDesc grid_view(const grayhip_ctx *c)
  * form's diagnostics are recomputed in FP64 at the end. */
 static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
 {
+	if (c->rt.n_freq > 0 && !grayhip_spacetime_has_rho(c->st))
+		return fail(c, GRAYHIP_ESTATE, "transfer needs the density of every level in use "
+		                               "(grayhip_spacetime_upload_rho)");
 	const GridDesc g = grid_view(c);
 	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
 	HCK(c, hipEventRecord(c->ev[0], c->stream));
 	for (size_t d = 0; d < n_dump; ++d) {
 		if (c->cfg.precision == GRAYHIP_FP64)
-			HCK(c, launch_grid_evolve_f64(g, one, c->d_data, c->d_info, c->d_steps, c->n_rays,
-			                              c->stream));
+			HCK(c, launch_grid_evolve_f64(g, c->rt, one, c->d_data, c->d_info, c->d_steps,
+			                              c->n_rays, c->stream));
 		else
-			HCK(c, launch_grid_evolve_f32(g, one, c->d_data, c->d_steps, c->n_rays,
+			HCK(c, launch_grid_evolve_f32(g, c->rt, one, c->d_data, c->d_steps, c->n_rays,
 			                              &c->d_ctr->pool_steps, c->stream));
 	}
 	HCK(c, hipEventRecord(c->ev[1], c->stream));

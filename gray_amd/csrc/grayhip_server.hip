@@ -238,6 +238,8 @@ extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q
 	if (!(fabs(q->a_spin) < 1.0) || q->n_sub == 0 || q->n_sub > (size_t)INT32_MAX ||
 	    q->n_dump == 0 || q->n_dump > (size_t)INT32_MAX || !(q->dt != 0.0))
 		return vfail(v, GRAYHIP_EINVAL, "bad spin or schedule");
+	if (q->n_freq > 0 && q->fluid != GRAYHIP_FLUID_DISK && q->fluid != GRAYHIP_FLUID_UNIFORM)
+		return vfail(v, GRAYHIP_EINVAL, "served analytic requests use the disk or uniform fluid");
 	if (q->spacetime && (q->spacetime->desc.dtype != 4 || q->n_freq != 0 ||
 	                     q->spacetime->desc.device != v->cfg.device))
 		return vfail(v, GRAYHIP_EINVAL, "gridded requests need an FP32 spacetime on this "

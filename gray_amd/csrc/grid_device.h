@@ -6,7 +6,8 @@
  *
  * HBM layout (per time level): Christoffels interleaved per grid point,
  * gam[p*40 + 10*mu + c] with c in tt tx ty tz xx xy xz yy yz zz, and the
- * metric met[p*10 + c]; p = (k*ny + j)*nx + i.  One RHS reads the 8 corner
+ * metric met[p*10 + c], and the density rho[p]; p = (k*ny + j)*nx + i.
+ * One RHS reads the 8 corner
  * points' 40 values (one contiguous 160 B / 320 B run each) from both
  * levels; the cell and its weights are computed once per RHS for all
  * fields (the reference recomputes the map for each of its 102 image reads).
@@ -24,6 +25,7 @@ template <class S>
 struct GridView {
 	const S *gam[2];      /* per level: [p][40]                        */
 	const S *met[2];      /* per level: [p][10]                        */
+	const S *rho[2];      /* per level: [p], density (may be null)     */
 	int    n[3];
 	int    map;           /* GRAYHIP_MAP_*                             */
 	double lo[3], hi[3];
@@ -158,6 +160,7 @@ namespace gray {
 struct GridDesc {
 	int dtype;                    /* 4 or 8 bytes per stored real          */
 	const void *gam[2], *met[2];  /* device arrays of the t1 / t2 slots     */
+	const void *rho[2];           /* densities of the slots (may be null)  */
 	int n[3];
 	int map;
 	double lo[3], hi[3], dx[3], fa[3], fb[3];
@@ -170,6 +173,7 @@ struct GridDesc {
 		for (int l = 0; l < 2; ++l) {
 			v.gam[l] = (const S *)gam[l];
 			v.met[l] = (const S *)met[l];
+			v.rho[l] = (const S *)rho[l];
 		}
 		for (int a = 0; a < 3; ++a) {
 			v.n[a] = n[a];

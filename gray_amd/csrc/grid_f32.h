@@ -1,11 +1,14 @@
 /*
  * grid_f32.h -- FP32 form of the gridded RHS: per corner, contract the 40
  * Christoffels with the ten products u^a u^b, then interpolate the four
- * results; RK4 on a compensated (hi + lo) state.
+ * results; RK4 on a compensated (hi + lo) state.  With transfer, the
+ * metric row g_{t nu} and the density are contracted and interpolated the
+ * same way.
  */
 #pragma once
 
 #include "grid_device.h"
+#include "rt_thermal.h"
 
 namespace gray {
 
@@ -58,8 +61,45 @@ __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const St
 	}
 }
 
+/* sum_k w_k (g_{t nu}(corner k) u^nu) and sum_k w_k rho(corner k): k_t and
+ * n_e of one level (interpolation commutes with the contraction). */
+__device__ __forceinline__ void load4(const float *__restrict__ a, float v[4])
+{
+	const float2 *a2 = reinterpret_cast<const float2 *>(a);   /* p*40 B: 8-byte aligned */
+	const float2 x = a2[0], y = a2[1];
+	v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+}
+
+__device__ __forceinline__ void load4(const double *__restrict__ a, float v[4])
+{
+	const double2 *a2 = reinterpret_cast<const double2 *>(a);
+	const double2 x = a2[0], y = a2[1];
+	v[0] = (float)x.x; v[1] = (float)x.y; v[2] = (float)y.x; v[3] = (float)y.y;
+}
+
 template <class S>
-__device__ __forceinline__ void grid_rhs_f32(const GridView<S> &G, const float *s, float *du)
+__device__ __forceinline__ void lev_flow(const S *__restrict__ met, const S *__restrict__ rho,
+                                         const Stencil<float> &st, const float u[4], float &kt,
+                                         float &ne)
+{
+	kt = 0.f;
+	ne = 0.f;
+#pragma unroll 1
+	for (int k = 0; k < 8; ++k) {
+		const uint32_t p = st.corner(k);
+		float m[4];
+		load4(met + (size_t)p * 10, m);
+		const float w = st.weight(k);
+		kt += w * (((m[0] * u[0] + m[1] * u[1]) + m[2] * u[2]) + m[3] * u[3]);
+		ne += w * (float)rho[p];
+	}
+}
+
+/* Geodesic RHS and, for NF > 0, the transfer of the reference flow on the
+ * scaled state (I' = I / K, tau' = tau / K). */
+template <int NF, class S>
+__device__ __forceinline__ void grid_rhs_all_f32(const GridView<S> &G, const RtThermalF &rt,
+                                                 const float *s, float *ds)
 {
 	Stencil<float> st;
 	make_stencil<float, S>(G, s[0], s[1], s[2], s[3], st);
@@ -76,42 +116,53 @@ __device__ __forceinline__ void grid_rhs_f32(const GridView<S> &G, const float *
 			uGu[mu] = uGu[mu] + st.wt * (y2[mu] - uGu[mu]);
 	}
 	const float u[4] = {u0, u1, u2, u3};
+	ds[0] = u0; ds[1] = u1; ds[2] = u2; ds[3] = u3;
 #pragma unroll
 	for (int mu = 0; mu < 4; ++mu)
-		du[mu] = -uGu[mu] + uGu[0] * u[mu];
+		ds[4 + mu] = -uGu[mu] + uGu[0] * u[mu];
+	if constexpr (NF > 0) {
+		float kt, ne;
+		lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
+		if (st.wt >= 0.f) {
+			float kt2, ne2;
+			lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
+			kt = kt + st.wt * (kt2 - kt);
+			ne = ne + st.wt * (ne2 - ne);
+		}
+		thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
+	}
 }
 
 /* Classical RK4 with stages from hi, increment added to (hi, lo). */
-template <class S>
-__device__ __forceinline__ void grid_rk4_f32c(const GridView<S> &G, float *hi, float *lo, float dt)
+template <int NF, class S>
+__device__ __forceinline__ void grid_rk4_f32c(const GridView<S> &G, const RtThermalF &rt,
+                                              float *hi, float *lo, float dt)
 {
-	float k[8], acc[8], x[8];
+	constexpr int ND = 8 + 2 * NF;
+	float k[ND], acc[ND], x[ND];
 	const float hdt = 0.5f * dt;
-#define GRHS(in, out) do { out[0] = in[4]; out[1] = in[5]; out[2] = in[6]; out[3] = in[7]; \
-		grid_rhs_f32<S>(G, in, out + 4); } while (0)
-	GRHS(hi, k);
+	grid_rhs_all_f32<NF, S>(G, rt, hi, k);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < ND; ++i) {
 		acc[i] = k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	GRHS(x, k);
+	grid_rhs_all_f32<NF, S>(G, rt, x, k);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	GRHS(x, k);
+	grid_rhs_all_f32<NF, S>(G, rt, x, k);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + dt * k[i];
 	}
-	GRHS(x, k);
-#undef GRHS
+	grid_rhs_all_f32<NF, S>(G, rt, x, k);
 	const float dt6 = dt * (1.0f / 6.0f);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < ND; ++i) {
 		float y = (acc[i] + k[i]) * dt6 + lo[i];
 		float t = hi[i] + y;
 		lo[i] = y - (t - hi[i]);

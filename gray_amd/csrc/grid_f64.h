@@ -6,6 +6,7 @@
 #pragma once
 
 #include "grid_device.h"
+#include "rt_thermal.h"
 
 namespace gray {
 
@@ -51,10 +52,9 @@ __device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
 }
 
 template <class S>
-__device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double *s, double *du)
+__device__ __forceinline__ void grid_rhs_at_f64(const GridView<S> &G, const Stencil<double> &st,
+                                                const double *s, double *du)
 {
-	Stencil<double> st;
-	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
 	const double u[4] = {s[4], s[5], s[6], s[7]};
 	double uGu[4];
 #pragma unroll 1
@@ -74,6 +74,34 @@ __device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double 
 #pragma unroll
 	for (int mu = 0; mu < 4; ++mu)
 		du[mu] = -uGu[mu] + uGu[0] * u[mu];
+}
+
+template <class S>
+__device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double *s, double *du)
+{
+	Stencil<double> st;
+	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
+	grid_rhs_at_f64<S>(G, st, s, du);
+}
+
+/* The whole state derivative: geodesic part, then (NF > 0) the transfer
+ * along the reference flow: n_e = interpolated density, shift = -k_t with
+ * k_t = g_{t nu} u^nu (fluid at rest; sim-org/flow.cl:50-54). */
+template <int NF, class S>
+__device__ __forceinline__ void grid_rhs_all_f64(const GridView<S> &G, const RtThermal &rt,
+                                                 const double *s, double *ds)
+{
+	Stencil<double> st;
+	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
+	ds[0] = s[4]; ds[1] = s[5]; ds[2] = s[6]; ds[3] = s[7];
+	grid_rhs_at_f64<S>(G, st, s, ds + 4);
+	if constexpr (NF > 0) {
+		double m[4], ne;
+		sample_f64<4, 10, S>(G.met, 0, st, m);
+		sample_f64<1, 1, S>(G.rho, 0, st, &ne);
+		const double kt = ((m[0] * s[4] + m[1] * s[5]) + m[2] * s[6]) + m[3] * s[7];
+		thermal_f64<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
+	}
 }
 
 template <class S>

@@ -130,20 +130,6 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 	}
 }
 
-#define GRAY_NF_SWITCH(NFV, CALL)                       \
-	switch (NFV) {                                  \
-	case 0: CALL(0); break;                         \
-	case 1: CALL(1); break;                         \
-	case 2: CALL(2); break;                         \
-	case 3: CALL(3); break;                         \
-	case 4: CALL(4); break;                         \
-	case 5: CALL(5); break;                         \
-	case 6: CALL(6); break;                         \
-	case 7: CALL(7); break;                         \
-	case 8: CALL(8); break;                         \
-	default: return hipErrorInvalidValue;           \
-	}
-
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks)
 {
 	*blocks = 0;

@@ -53,20 +53,6 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-#define GRAY_NF_SWITCH(NFV, CALL)                       \
-	switch (NFV) {                                  \
-	case 0: CALL(0); break;                         \
-	case 1: CALL(1); break;                         \
-	case 2: CALL(2); break;                         \
-	case 3: CALL(3); break;                         \
-	case 4: CALL(4); break;                         \
-	case 5: CALL(5); break;                         \
-	case 6: CALL(6); break;                         \
-	case 7: CALL(7); break;                         \
-	case 8: CALL(8); break;                         \
-	default: return hipErrorInvalidValue;           \
-	}
-
 hipError_t launch_fallback_f64(const RayParams &p, const StepRule &sr, double *data, double *info,
                                int64_t *steps, size_t n_rays, const Fallback *fb,
                                Counters *ctr, unsigned max_fb, hipStream_t st)

@@ -15,30 +15,29 @@
 
 namespace gray {
 
-/* sim-org/RK4.cl:37-47 on the geodesic state. */
-template <class S>
-__device__ __forceinline__ void grid_rk4_f64(const GridView<S> &G, double *s, double dt)
+/* sim-org/RK4.cl:37-47 on the whole state (8 + 2 NF components). */
+template <int NF, class S>
+__device__ __forceinline__ void grid_rk4_f64(const GridView<S> &G, const RtThermal &rt, double *s,
+                                             double dt)
 {
-	double k1[8], k2[8], k[8], x[8];
+	constexpr int ND = 8 + 2 * NF;
+	double k1[ND], k2[ND], k[ND], x[ND];
 	const double hdt = 0.5 * dt;
-#define GRHS(in, out) do { out[0] = in[4]; out[1] = in[5]; out[2] = in[6]; out[3] = in[7]; \
-		grid_rhs_f64<S>(G, in, out + 4); } while (0)
-	GRHS(s, k1);
+	grid_rhs_all_f64<NF, S>(G, rt, s, k1);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k1[i];
-	GRHS(x, k2);
+	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
+	grid_rhs_all_f64<NF, S>(G, rt, x, k2);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) x[i] = s[i] + hdt * k2[i];
-	GRHS(x, k);
+	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k2[i];
+	grid_rhs_all_f64<NF, S>(G, rt, x, k);
 #pragma unroll
-	for (int i = 0; i < 8; ++i) {
+	for (int i = 0; i < ND; ++i) {
 		k2[i] = k2[i] + k[i];
 		x[i] = s[i] + dt * k[i];
 	}
-	GRHS(x, k);
-#undef GRHS
+	grid_rhs_all_f64<NF, S>(G, rt, x, k);
 #pragma unroll
-	for (int i = 0; i < 8; ++i)
+	for (int i = 0; i < ND; ++i)
 		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k2[i]) + k[i])) / 6.0;
 }
 
@@ -46,7 +45,8 @@ template <class S>
 __global__ __launch_bounds__(256) void k_grid_icond(GridView<S> G, RayParams p,
                                                     double *__restrict__ data,
                                                     double *__restrict__ info,
-                                                    int64_t *__restrict__ steps, size_t n_rays)
+                                                    int64_t *__restrict__ steps, size_t n_rays,
+                                                    int nd)
 {
 	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= n_rays)
@@ -77,22 +77,25 @@ __global__ __launch_bounds__(256) void k_grid_icond(GridView<S> G, RayParams p,
 #pragma unroll
 	for (int i = 0; i < 8; ++i)
 		data[(size_t)i * n_rays + g] = s[i];
+	for (int i = 8; i < nd; ++i)
+		data[(size_t)i * n_rays + g] = 0.0;   /* rt_icond, sim-org/rt.cl:179-183 */
 	info[g] = grid_getuu_f64<S>(G, s);
 	steps[g] = 0;
 }
 
-template <class S>
-__global__ __launch_bounds__(256) void k_grid_evolve_f64(GridView<S> G, StepRule sr,
+template <int NF, class S>
+__global__ __launch_bounds__(256) void k_grid_evolve_f64(GridView<S> G, RtThermal rt, StepRule sr,
                                                          double *__restrict__ data,
                                                          double *__restrict__ info,
                                                          int64_t *__restrict__ steps, size_t n_rays)
 {
+	constexpr int ND = 8 + 2 * NF;
 	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= n_rays)
 		return;
-	double s[8];
+	double s[ND];
 #pragma unroll
-	for (int i = 0; i < 8; ++i)
+	for (int i = 0; i < ND; ++i)
 		s[i] = data[(size_t)i * n_rays + g];
 	int64_t nst = 0;
 	int dh;
@@ -103,11 +106,11 @@ __global__ __launch_bounds__(256) void k_grid_evolve_f64(GridView<S> G, StepRule
 			break;
 		if (dh > sr.n - h)
 			dh = sr.n - h;
-		grid_rk4_f64<S>(G, s, (double)dh * sr.dt / (double)sr.n);
+		grid_rk4_f64<NF, S>(G, rt, s, (double)dh * sr.dt / (double)sr.n);
 		++nst;
 	}
 #pragma unroll
-	for (int i = 0; i < 8; ++i)
+	for (int i = 0; i < ND; ++i)
 		data[(size_t)i * n_rays + g] = s[i];
 	info[g] = grid_getuu_f64<S>(G, s);
 	steps[g] += nst;
@@ -140,8 +143,10 @@ __global__ void k_grid_sample_f64(GridView<S> G, const double *__restrict__ ev, 
 	double v[1];
 	if (which == 0)
 		sample_f64<1, 40, S>(G.gam, comp, st, v);
-	else
+	else if (which == 1)
 		sample_f64<1, 10, S>(G.met, comp, st, v);
+	else
+		sample_f64<1, 1, S>(G.rho, 0, st, v);
 	out[i] = v[0];
 }
 
@@ -152,26 +157,38 @@ hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *dat
 {
 	if (n_rays == 0)
 		return hipSuccess;
+	const int nd = 8 + 2 * p.n_freq;
 	if (gd.dtype == 4)
 		hipLaunchKernelGGL(k_grid_icond<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
-		                   gd.view<float>(), p, data, info, steps, n_rays);
+		                   gd.view<float>(), p, data, info, steps, n_rays, nd);
 	else
 		hipLaunchKernelGGL(k_grid_icond<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
-		                   gd.view<double>(), p, data, info, steps, n_rays);
+		                   gd.view<double>(), p, data, info, steps, n_rays, nd);
 	return hipGetLastError();
 }
 
-hipError_t launch_grid_evolve_f64(const GridDesc &gd, const StepRule &sr, double *data, double *info,
-                                  int64_t *steps, size_t n_rays, hipStream_t st)
+template <int NF>
+static void grid_evolve_f64_nf(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                               double *data, double *info, int64_t *steps, size_t n_rays,
+                               hipStream_t st)
+{
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL((k_grid_evolve_f64<NF, float>), dim3(blocks_for(n_rays, 256)), dim3(256),
+		                   0, st, gd.view<float>(), rt, sr, data, info, steps, n_rays);
+	else
+		hipLaunchKernelGGL((k_grid_evolve_f64<NF, double>), dim3(blocks_for(n_rays, 256)), dim3(256),
+		                   0, st, gd.view<double>(), rt, sr, data, info, steps, n_rays);
+}
+
+hipError_t launch_grid_evolve_f64(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                  double *data, double *info, int64_t *steps, size_t n_rays,
+                                  hipStream_t st)
 {
 	if (n_rays == 0)
 		return hipSuccess;
-	if (gd.dtype == 4)
-		hipLaunchKernelGGL(k_grid_evolve_f64<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
-		                   gd.view<float>(), sr, data, info, steps, n_rays);
-	else
-		hipLaunchKernelGGL(k_grid_evolve_f64<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
-		                   gd.view<double>(), sr, data, info, steps, n_rays);
+#define L(NF) grid_evolve_f64_nf<NF>(gd, rt, sr, data, info, steps, n_rays, st)
+	GRAY_NF_SWITCH(rt.n_freq, L)
+#undef L
 	return hipGetLastError();
 }
 

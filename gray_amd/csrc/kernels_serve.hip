@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_srv_grid32(const SrvRequest *__restrict
 		}
 		if (dh > sr.n - h)
 			dh = sr.n - h;
-		grid_rk4_f32c<float>(G, hi, lo, (float)((double)dh * dt_over_n));
+		grid_rk4_f32c<0, float>(G, RtThermalF{}, hi, lo, (float)((double)dh * dt_over_n));
 		++st;
 		h += dh;
 		if (h >= sr.n) {

@@ -8,8 +8,25 @@
 
 #include "grayhip_params.h"
 #include "grid_device.h"
+#include "rt_thermal.h"
 
 namespace gray {
+
+/* Instantiate CALL(n) for the run-time channel count NFV (0..8) inside a
+ * launcher returning hipError_t. */
+#define GRAY_NF_SWITCH(NFV, CALL)                       \
+	switch (NFV) {                                  \
+	case 0: CALL(0); break;                         \
+	case 1: CALL(1); break;                         \
+	case 2: CALL(2); break;                         \
+	case 3: CALL(3); break;                         \
+	case 4: CALL(4); break;                         \
+	case 5: CALL(5); break;                         \
+	case 6: CALL(6); break;                         \
+	case 7: CALL(7); break;                         \
+	case 8: CALL(8); break;                         \
+	default: return hipErrorInvalidValue;           \
+	}
 
 hipError_t launch_icond(const RayParams &p, double *data, double *info, int64_t *steps,
                         size_t n_rays, hipStream_t st);
@@ -33,14 +50,15 @@ hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data,
 
 hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *data, double *info,
                              int64_t *steps, size_t n_rays, hipStream_t st);
-hipError_t launch_grid_evolve_f64(const GridDesc &gd, const StepRule &sr, double *data, double *info,
-                                  int64_t *steps, size_t n_rays, hipStream_t st);
+hipError_t launch_grid_evolve_f64(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                  double *data, double *info, int64_t *steps, size_t n_rays,
+                                  hipStream_t st);
 hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *info, size_t n_rays,
                                 hipStream_t st);
 hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,
                                   double *out, int n, hipStream_t st);
-hipError_t launch_grid_evolve_f32(const GridDesc &gd, const StepRule &sr, double *data,
-                                  int64_t *steps, size_t n_rays, unsigned long long *nsteps,
-                                  hipStream_t st);
+hipError_t launch_grid_evolve_f32(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                  double *data, int64_t *steps, size_t n_rays,
+                                  unsigned long long *nsteps, hipStream_t st);
 
 } // namespace gray

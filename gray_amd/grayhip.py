@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libgrayhip.so")
 MAX_FREQ = 8
 FP32, FP64 = 4, 8
 ESC_REF, ESC_OUTWARD = 0, 1
-FLUID_NONE, FLUID_DISK, FLUID_UNIFORM = 0, 1, 2
+FLUID_NONE, FLUID_DISK, FLUID_UNIFORM, FLUID_THERMAL = 0, 1, 2, 3
 
 # Every entry point declared in include/grayhip.h.
 EXPORTS = (
@@ -25,7 +25,7 @@ EXPORTS = (
     "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
     "grayhip_spacetime_create", "grayhip_spacetime_destroy", "grayhip_spacetime_upload",
     "grayhip_spacetime_shift", "grayhip_spacetime_set_frozen", "grayhip_spacetime_bytes",
-    "grayhip_spacetime_sample", "grayhip_spacetime_error",
+    "grayhip_spacetime_sample", "grayhip_spacetime_error", "grayhip_spacetime_upload_rho",
 )
 MAP_UNIFORM, MAP_FISHEYE = 0, 1
 
@@ -57,6 +57,7 @@ class Config(C.Structure):
         ("spacetime", C.c_void_p),
         ("t_init", C.c_double),
         ("raw_icond", C.c_int),
+        ("m_adm", C.c_double),
     ]
 
 
@@ -104,6 +105,7 @@ def lib():
     L.grayhip_spacetime_bytes.argtypes = [vp]
     L.grayhip_spacetime_bytes.restype = C.c_size_t
     L.grayhip_spacetime_sample.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, vp]
+    L.grayhip_spacetime_upload_rho.argtypes = [vp, C.c_int, vp]
     L.grayhip_spacetime_error.argtypes = [vp]
     L.grayhip_spacetime_error.restype = C.c_char_p
     _lib = L
@@ -144,6 +146,11 @@ class Spacetime:
         assert gam.size == 40 * self.n_points and met.size == 10 * self.n_points
         self._check(lib().grayhip_spacetime_upload(self._h, slot, time, _ptr(gam), _ptr(met)),
                     "upload")
+
+    def upload_rho(self, slot, rho):
+        rho = np.ascontiguousarray(rho, dtype=self.np_dtype)
+        assert rho.size == self.n_points
+        self._check(lib().grayhip_spacetime_upload_rho(self._h, slot, _ptr(rho)), "upload_rho")
 
     def shift(self, to_t1=True):
         self._check(lib().grayhip_spacetime_shift(self._h, int(to_t1)), "shift")
@@ -188,7 +195,7 @@ class Render:
 
     def __init__(self, w_rays, h_rays, a_spin=0.0, precision=FP64, nu=(), fluid=None,
                  escape=ESC_OUTWARD, r_esc=1000.0, eps_stop=0.0, device=0,
-                 spacetime=None, t_init=0.0, raw_icond=False):
+                 spacetime=None, t_init=0.0, raw_icond=False, m_adm=0.0):
         cfg = Config()
         cfg.w_rays, cfg.h_rays = w_rays, h_rays
         cfg.precision = precision
@@ -197,7 +204,8 @@ class Render:
         for i, v in enumerate(nu):
             cfg.nu[i] = v
         if fluid is None:
-            fluid = FLUID_DISK if len(nu) else FLUID_NONE
+            fluid = (FLUID_THERMAL if spacetime is not None else FLUID_DISK) if len(nu) \
+                else FLUID_NONE
         cfg.fluid = fluid
         cfg.escape = escape
         cfg.r_esc = r_esc
@@ -206,6 +214,7 @@ class Render:
         cfg.spacetime = spacetime.handle if spacetime is not None else None
         cfg.t_init = t_init
         cfg.raw_icond = int(raw_icond)
+        cfg.m_adm = m_adm
         self._spacetime = spacetime  # keep it alive while the render exists
         self.cfg = cfg
         self.n_rays = w_rays * h_rays

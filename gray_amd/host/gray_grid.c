@@ -37,6 +37,7 @@ struct gray_grid {
 	size_t snap;         /* level in slot 0 */
 	int frozen;
 	void *gam, *met;     /* host staging of one level */
+	void *rho;           /* and its density (transfer only) */
 	char err[512];
 };
 
@@ -56,9 +57,9 @@ const char *gray_grid_error(const gray_grid *g) { return g ? g->err : g_err; }
 int gray_grid_create(const grayhip_config *cfg, gray_grid **out)
 {
 	*out = NULL;
-	if (cfg->n_freq > 0)
-		return gfail(NULL, "radiative transfer on a numerical spacetime is not supported "
-		                   "(nu must be empty with coordinates=dyst)");
+	if (cfg->n_freq > 0 && cfg->fluid != GRAYHIP_FLUID_THERMAL)
+		return gfail(NULL, "radiative transfer on a numerical spacetime uses the reference "
+		                   "flow (fluid=thermal)");
 	gray_grid *g = calloc(1, sizeof(gray_grid));
 	if (!g)
 		return gfail(NULL, "out of memory");
@@ -79,6 +80,7 @@ void gray_grid_destroy(gray_grid *g)
 	free(g->times);
 	free(g->gam);
 	free(g->met);
+	free(g->rho);
 	free(g);
 }
 
@@ -96,10 +98,13 @@ static size_t find_snapshot(const gray_grid *g, double t)
 
 static int load_level(gray_grid *g, size_t level, int slot)
 {
-	if (gray_h5_read_level(g->path, g->names[level], g->n, g->gam, g->met, NULL, g->dtype))
+	if (gray_h5_read_level(g->path, g->names[level], g->n, g->gam, g->met, g->rho, g->dtype))
 		return gfail(g, "%s", gray_h5_error());
 	if (grayhip_spacetime_upload(g->st, slot, g->times[level], g->gam, g->met))
 		return gfail(g, "upload of level %s: %s", g->names[level], grayhip_spacetime_error(g->st));
+	if (g->rho && grayhip_spacetime_upload_rho(g->st, slot, g->rho))
+		return gfail(g, "density of level %s: %s", g->names[level],
+		             grayhip_spacetime_error(g->st));
 	return 0;
 }
 
@@ -158,6 +163,8 @@ int gray_grid_load(gray_grid *g, const char *path, double t_init, double dt_dump
 	g->gam = malloc(np * 40 * (size_t)g->dtype);
 	g->met = malloc(np * 10 * (size_t)g->dtype);
 	if (!g->gam || !g->met)
+		return gfail(g, "out of memory");
+	if (g->cfg.n_freq > 0 && !(g->rho = malloc(np * (size_t)g->dtype)))
 		return gfail(g, "out of memory");
 	if (grayhip_spacetime_create(&d, &g->st))
 		return gfail(g, "cannot allocate the spacetime on the device");

@@ -57,10 +57,14 @@ static int _conf(void *ego, const char *arg)
 	return 0;
 }
 
-static int pick_fluid(const char *s, int *out)
+static int pick_fluid(const char *s, int analytic, int *out)
 {
-	if (!s || !strcmp(s, "disk"))
+	if (!s || !*s)
+		*out = analytic ? GRAYHIP_FLUID_DISK : GRAYHIP_FLUID_THERMAL;
+	else if (!strcmp(s, "disk"))
 		*out = GRAYHIP_FLUID_DISK;
+	else if (!strcmp(s, "thermal"))
+		*out = GRAYHIP_FLUID_THERMAL;
 	else if (!strcmp(s, "uniform"))
 		*out = GRAYHIP_FLUID_UNIFORM;
 	else if (!strcmp(s, "none"))
@@ -103,10 +107,12 @@ static int _init(void *ego)
 	cfg.n_freq = (int)EGO->n_freq;
 	for (size_t i = 0; i < EGO->n_freq; ++i)
 		cfg.nu[i] = o->nu[i];
-	if (pick_fluid(o->fluid, &cfg.fluid)) {
-		fprintf(stderr, "GRay2: unknown fluid \"%s\" (disk, uniform, none)\n", o->fluid);
+	if (pick_fluid(o->fluid, analytic(o), &cfg.fluid)) {
+		fprintf(stderr, "GRay2: unknown fluid \"%s\" (disk, uniform, thermal, none)\n",
+		        o->fluid);
 		return EXIT_FAILURE;
 	}
+	cfg.m_adm = o->M_ADM;
 	if (!EGO->n_freq)
 		cfg.fluid = GRAYHIP_FLUID_NONE;
 	if (!o->escape || !strcmp(o->escape, "outward"))

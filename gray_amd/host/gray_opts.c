@@ -78,7 +78,9 @@ static const opt_row rows[] = {
 	 "Termination: outward (r<1.01 r_H, or r>r_esc moving outward) or ref (as phys.cl)"},
 	{"r_esc", T_REAL, OFF(r_esc), "1000", "build", "Escape radius of the outward rule"},
 	{"eps_stop", T_REAL, OFF(eps_stop), "0", "build", "Horizon stop r - r_H < eps_stop (0: rule default)"},
-	{"fluid", T_STR, OFF(fluid), "disk", "build", "Fluid for radiative transfer: disk, uniform or none"},
+	{"fluid", T_STR, OFF(fluid), "", "build",
+	 "Fluid for radiative transfer: disk (analytic default), thermal (the reference flow "
+	 "on a numerical spacetime, its default), uniform or none"},
 };
 
 #define N_ROWS (sizeof(rows) / sizeof(rows[0]))

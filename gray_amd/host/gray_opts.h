@@ -43,7 +43,7 @@ struct gray_options {
 	char  *escape;              /* "outward" (r<1.01 r_H or r>r_esc outward) | "ref" */
 	double r_esc;
 	double eps_stop;            /* <= 0: rule default */
-	char  *fluid;               /* "disk" | "uniform" | "none" */
+	char  *fluid;               /* "disk" | "thermal" | "uniform" | "none"; "" by coordinates */
 };
 
 /* Fill defaults (lux-optgen's X_init). */

@@ -48,11 +48,18 @@ enum {
 	                             the integration direction                */
 };
 
-/* Fluid feeding radiative transfer (sim-org/flow.cl:36-58 replaced). */
+/* Fluid and emission model feeding radiative transfer. */
 enum {
 	GRAYHIP_FLUID_NONE    = 0,
-	GRAYHIP_FLUID_DISK    = 1,  /* synthetic Keplerian disk             */
-	GRAYHIP_FLUID_UNIFORM = 2   /* n = 1, shift = 1 (slab closed forms) */
+	GRAYHIP_FLUID_DISK    = 1,  /* synthetic Keplerian disk, j = n nu^-1/2,
+	                               alpha = 0.1 n nu^-5/2 (analytic Kerr)   */
+	GRAYHIP_FLUID_UNIFORM = 2,  /* n = 1, shift = 1 (slab closed forms)     */
+	GRAYHIP_FLUID_THERMAL = 3   /* the reference model, sim-org/flow.cl:36-58
+	                               with rt.cl:80-198: n_e from the
+	                               spacetime's density, T_e = 1e12, b = 0,
+	                               fluid at rest; bremsstrahlung emission
+	                               (synchrotron vanishes at b = 0) and
+	                               Kirchhoff absorption.  Gridded only.    */
 };
 
 /* Grid coordinate maps of a numerical spacetime (sim-org/interp.cl:69-121
@@ -88,11 +95,14 @@ typedef struct {
 	                            0.01 r_H for ESC_OUTWARD, 0.01 for ESC_REF  */
 	int    device;           /* HIP device ordinal                          */
 	/* numerical spacetime (coordinates=dyst); NULL: analytic Kerr-Schild.
-	 * With a spacetime, n_freq must be 0, `escape` selects box-exit
-	 * termination (ESC_OUTWARD) or none (ESC_REF, as sim-org/dyst.cl). */
+	 * With a spacetime, transfer needs GRAYHIP_FLUID_THERMAL and uploaded
+	 * densities; `escape` selects box-exit termination with the horizon
+	 * proxy (ESC_OUTWARD) or none (ESC_REF, as sim-org/dyst.cl). */
 	grayhip_spacetime *spacetime;
 	double t_init;           /* grid time of the rays' q^t = 0              */
 	int    raw_icond;        /* 1: no null normalisation (dyst.cl:89-110)   */
+	double m_adm;            /* M_ADM in solar masses for FLUID_THERMAL
+	                            (sim-org/param.opts:4); <= 0 picks 4.3e6    */
 } grayhip_config;
 
 typedef struct grayhip_ctx grayhip_ctx;
@@ -166,6 +176,10 @@ void grayhip_spacetime_destroy(grayhip_spacetime *st);
  * 0 (t1) or 1 (t2) at the given time.  Blocks until the copy is done. */
 int  grayhip_spacetime_upload(grayhip_spacetime *st, int slot, double time,
                               const void *gam, const void *met);
+/* Copy the density (the rho dataset, n_points reals of d->dtype in the
+ * same point order) of the level held in `slot`; GRAYHIP_FLUID_THERMAL
+ * reads it as n_e (sim-org/flow.cl:50). */
+int  grayhip_spacetime_upload_rho(grayhip_spacetime *st, int slot, const void *rho);
 /* Slide the window: to_t1 != 0 makes the t2 slot the new t1 slot, else the
  * t1 slot the new t2 slot (sim-org/io.c:279-300); the other slot is then
  * free for the next upload. */
@@ -175,7 +189,8 @@ int  grayhip_spacetime_set_frozen(grayhip_spacetime *st, int frozen);
 /* Device bytes held by the spacetime. */
 size_t grayhip_spacetime_bytes(const grayhip_spacetime *st);
 /* Interpolate one field at n events (t,x,y,z) on the device in FP64:
- * which 0: Christoffel component comp < 40, 1: metric comp < 10. */
+ * which 0: Christoffel component comp < 40, 1: metric comp < 10,
+ * 2: density (comp 0). */
 int  grayhip_spacetime_sample(grayhip_spacetime *st, int n, const double *events,
                               int which, int comp, double t_init, double *out);
 const char *grayhip_spacetime_error(const grayhip_spacetime *st);

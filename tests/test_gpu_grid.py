@@ -147,6 +147,11 @@ def test_spacetime_errors():
     st = G.Spacetime((4, 4, 4), (-1,) * 3, (1,) * 3)
     with pytest.raises(G.GrayHipError):
         st.set_frozen(False)  # only one level could be loaded
-    with pytest.raises(G.GrayHipError):
-        G.Render(4, 4, nu=(1.0,), spacetime=st)  # no transfer on grids
+    with pytest.raises(G.GrayHipError):   # gridded transfer uses the reference flow
+        G.Render(4, 4, nu=(1e11,), spacetime=st, fluid=G.FLUID_DISK)
+    with pytest.raises(G.GrayHipError):   # the thermal flow needs a numerical spacetime
+        G.Render(4, 4, nu=(1e11,), fluid=G.FLUID_THERMAL)
+    with G.Render(4, 4, nu=(1e11,), spacetime=st) as r:
+        with pytest.raises(G.GrayHipError):   # no density uploaded
+            r.icond(1.0, 1.0, 0.5, 45.0, 0.0)
     st.close()

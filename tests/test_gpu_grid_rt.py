@@ -1,0 +1,137 @@
+"""Radiative transfer on numerical spacetimes with the reference's flow and
+emission model (sim-org/flow.cl:36-58, rt.cl:80-198) on the GPU, through
+the C ABI.
+
+Bars:
+  * plain FP64 form against the FP64 grid oracle: geodesic state and step
+    counts bit for bit; I and tau to 1e-10 relative (exp and log are not
+    correctly rounded on either side);
+  * flat grid (Minkowski metric, zero Christoffels, constant density): the
+    closed forms I = j_ff L and tau = j_ff L / B_nu at path length L, in both
+    forms (FP64 1e-12, FP32 form 1e-5 relative);
+  * FP32 form (FP32 grid, state scaled by K = M_ADM ff_f^2) against the FP64
+    form: image relative L1 per channel <= 1e-4.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gray_amd import grayhip as G
+from gray_amd import job as J
+from gray_amd import spacetime as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOX = 40.0
+NUS = (1e11, 2.3e11, 5e11, 8e11)
+
+
+def boosted(n, times, dtype):
+    xs = S.axis(-BOX, BOX, n)
+    b = S.bks(1.0, 0.7, 0.3, (1, 0, 0))
+    lv = [S.bks_sample(b, t, xs, xs, xs, rho=True) for t in times]
+    tp = np.float32 if dtype == 4 else np.float64
+    return [tuple(a.astype(tp) for a in l) for l in lv]
+
+
+def device(n, lv, times, dtype, lo=-BOX, hi=BOX):
+    st = G.Spacetime((n,) * 3, (lo,) * 3, (hi,) * 3, dtype=dtype)
+    for slot, (g, m, r) in enumerate(lv):
+        st.upload(slot, times[slot], g, m)
+        st.upload_rho(slot, r)
+    st.set_frozen(len(lv) == 1)
+    return st
+
+
+def gpu_render(st, precision, w, nus=NUS, t_init=0.0, n_dump=2, cam=(32.0, 35.0, 45.0),
+               dt=-32.0, n_sub=1024):
+    with G.Render(w, w, precision=precision, spacetime=st, nu=nus, t_init=t_init) as r:
+        r.icond(cam[0], cam[0], cam[1], cam[2], 0.0)
+        for _ in range(n_dump):
+            r.evolve(dt, n_sub)
+        d, i = r.read_f64()
+        return d, i, r.steps()
+
+
+@pytest.mark.parametrize("dtype,times", [(8, (0.0,)), (4, (0.0, 1.0))])
+def test_thermal_fp64_matches_oracle(dtype, times):
+    n = 33
+    lv = boosted(n, times, dtype)
+    st = device(n, lv, times, dtype)
+    t_init = times[-1]
+    d1, i1, s1 = gpu_render(st, G.FP64, 12, t_init=t_init)
+    st.close()
+    og = O.GridSpacetime((n,) * 3, (-BOX,) * 3, (BOX,) * 3,
+                         [l[0].astype(np.float64) for l in lv], [l[1].astype(np.float64) for l in lv],
+                         times=(times[0], times[-1]), t_off=t_init,
+                         rho=[l[2].astype(np.float64) for l in lv], nu=NUS)
+    d0, i0, s0 = og.render(12, 12, 32.0, 32.0, 35.0, 45.0, 0.0, n_dump=2)
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0[:, :8], d1[:, :8])
+    assert np.array_equal(i0, i1)
+    rt0, rt1 = d0[:, 8:], d1[:, 8:]
+    assert (np.abs(rt0) > 0).any()
+    assert np.all(np.abs(rt1 - rt0) <= 1e-10 * np.abs(rt0)), np.abs(rt1 - rt0).max()
+
+
+@pytest.mark.parametrize("precision,tol", [(G.FP64, 1e-12), (G.FP32, 1e-5)])
+def test_flat_slab_closed_form(precision, tol):
+    n, L = 9, 10.0
+    xs = np.linspace(-L, L, n)
+    npts = n ** 3
+    gam = np.zeros((npts, 40))
+    met = np.zeros((npts, 10))
+    met[:, 0], met[:, 4], met[:, 7], met[:, 9] = -1.0, 1.0, 1.0, 1.0
+    rho = np.full(npts, 0.6)
+    dtype = 8 if precision == G.FP64 else 4
+    tp = np.float64 if dtype == 8 else np.float32
+    st = device(n, [(gam.astype(tp), met.astype(tp), rho.astype(tp))], (0.0,), dtype, -L, L)
+    d, _, s = gpu_render(st, precision, 8, cam=(4.0, 5.0, 45.0), dt=-4.0, n_sub=64, n_dump=6)
+    st.close()
+    rt = O.Rt()
+    path = -d[:, 0]                          # u^t = 1: affine length = -q^t
+    assert (path > 4.0).all()
+    for i, v in enumerate(NUS):
+        j = rt.j_ff(v, 1e12, 0.6)
+        B = rt.b_planck(v, 1e12)
+        np.testing.assert_allclose(d[:, 8 + i], j * path, rtol=tol)
+        np.testing.assert_allclose(d[:, 8 + len(NUS) + i], j / B * path, rtol=tol)
+
+
+def test_thermal_fp32_form_vs_fp64():
+    n, times = 33, (0.0,)
+    lv64 = boosted(n, times, 8)
+    lv32 = [tuple(a.astype(np.float32) for a in l) for l in lv64]
+    st64, st32 = device(n, lv64, times, 8), device(n, lv32, times, 4)
+    d64, _, s64 = gpu_render(st64, G.FP64, 32, n_dump=3)
+    d32, _, s32 = gpu_render(st32, G.FP32, 32, n_dump=3)
+    st64.close()
+    st32.close()
+    nf = len(NUS)
+    for blk in (slice(8, 8 + nf), slice(8 + nf, 8 + 2 * nf)):
+        a, b = d32[:, blk], d64[:, blk]
+        l1 = np.abs(a - b).sum(axis=0) / np.abs(b).sum(axis=0)
+        assert (l1 <= 1e-4).all(), l1
+
+
+def test_launcher_dyst_job_with_transfer(tmp_path):
+    exe = os.path.join(os.path.dirname(J.LIB_PATH), "gray-mkgrid")
+    f = str(tmp_path / "bks.h5")
+    r = subprocess.run([exe, f"out={f}", "n=33", "a=0.7", "v=0.3", "axis=x", "dtype=8",
+                        "times=0.0"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([J.LAUNCHER, "gray", "coor=dyst", f"data={f}", "W=8", "H=8", "w=32",
+                        "h=32", "r=35", "i=45", "p=8", "N=2", "nu=2.3e11,5e11",
+                        f"out={tmp_path}/%04d.raw"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g, m, rho = S.h5_read_level(f, "0.0", (33, 33, 33), rho=True)
+    og = O.GridSpacetime((33,) * 3, (-40.0,) * 3, (40.0,) * 3, [g], [m], rho=[rho],
+                         nu=(2.3e11, 5e11))
+    d0, i0, s0 = og.render(8, 8, 32.0, 32.0, 35.0, 45.0, 0.0, n_dump=2)
+    states, diag = J.load_raw(f"{tmp_path}/0002.raw")
+    states = states.reshape(-1, 12)
+    assert np.array_equal(states[:, :8], d0[:, :8])
+    assert np.all(np.abs(states[:, 8:] - d0[:, 8:]) <= 1e-10 * np.abs(d0[:, 8:]))
