@@ -29,12 +29,27 @@ BOX = 40.0
 NUS = (1e11, 2.3e11, 5e11, 8e11)
 
 
-def boosted(n, times, dtype):
+def boosted(n, times, dtype, rho_cap=100.0):
+    """Boosted Kerr-Schild levels with the generator's density 1/|x|, capped
+    at rho_cap: uncapped, the grid point at the origin holds ~1.8e308 (the
+    generator's nan_to_num of 1/0), and rays through its cells carry
+    j ~ n_e^2 = inf."""
     xs = S.axis(-BOX, BOX, n)
     b = S.bks(1.0, 0.7, 0.3, (1, 0, 0))
     lv = [S.bks_sample(b, t, xs, xs, xs, rho=True) for t in times]
+    if rho_cap:
+        lv = [(g, m, np.minimum(r, rho_cap)) for g, m, r in lv]
     tp = np.float32 if dtype == 4 else np.float64
     return [tuple(a.astype(tp) for a in l) for l in lv]
+
+
+def same_rt(rt0, rt1, rel=1e-10):
+    """Equal non-finite pattern; finite values within rel."""
+    fin0, fin1 = np.isfinite(rt0), np.isfinite(rt1)
+    assert np.array_equal(fin0, fin1)
+    assert np.array_equal(rt0[~fin0], rt1[~fin1], equal_nan=True)
+    a, b = rt0[fin0], rt1[fin1]
+    assert np.all(np.abs(b - a) <= rel * np.abs(a)), np.abs(b - a).max()
 
 
 def device(n, lv, times, dtype, lo=-BOX, hi=BOX):
@@ -56,10 +71,11 @@ def gpu_render(st, precision, w, nus=NUS, t_init=0.0, n_dump=2, cam=(32.0, 35.0,
         return d, i, r.steps()
 
 
-@pytest.mark.parametrize("dtype,times", [(8, (0.0,)), (4, (0.0, 1.0))])
-def test_thermal_fp64_matches_oracle(dtype, times):
+@pytest.mark.parametrize("dtype,times,cap", [(8, (0.0,), 100.0), (4, (0.0, 1.0), 100.0),
+                                             (8, (0.0,), None)])
+def test_thermal_fp64_matches_oracle(dtype, times, cap):
     n = 33
-    lv = boosted(n, times, dtype)
+    lv = boosted(n, times, dtype, cap)
     st = device(n, lv, times, dtype)
     t_init = times[-1]
     d1, i1, s1 = gpu_render(st, G.FP64, 12, t_init=t_init)
@@ -74,7 +90,9 @@ def test_thermal_fp64_matches_oracle(dtype, times):
     assert np.array_equal(i0, i1)
     rt0, rt1 = d0[:, 8:], d1[:, 8:]
     assert (np.abs(rt0) > 0).any()
-    assert np.all(np.abs(rt1 - rt0) <= 1e-10 * np.abs(rt0)), np.abs(rt1 - rt0).max()
+    if cap:
+        assert np.isfinite(rt0).all()
+    same_rt(rt0, rt1)
 
 
 @pytest.mark.parametrize("precision,tol", [(G.FP64, 1e-12), (G.FP32, 1e-5)])
@@ -134,4 +152,4 @@ def test_launcher_dyst_job_with_transfer(tmp_path):
     states, diag = J.load_raw(f"{tmp_path}/0002.raw")
     states = states.reshape(-1, 12)
     assert np.array_equal(states[:, :8], d0[:, :8])
-    assert np.all(np.abs(states[:, 8:] - d0[:, 8:]) <= 1e-10 * np.abs(d0[:, 8:]))
+    same_rt(d0[:, 8:], states[:, 8:])
