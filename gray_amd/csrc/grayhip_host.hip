@@ -358,7 +358,14 @@ static int make_rule(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, Ste
 static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 {
 	const int W = (int)c->cfg.w_rays, H = (int)c->cfg.h_rays;
-	const int r0 = H / 4, r1 = H - H / 4;
+	/* Pass 1: the rows within 8 M of the image centre, where the photon
+	 * ring and the shadow edge lie for every spin and inclination (critical
+	 * impact parameters 2-7 M); the rays whose steps collapse come from
+	 * there, so their FP64 re-integration overlaps pass 2. */
+	const double band = 8.0 * (double)H / fabs(c->p.h_img > 0.0 ? c->p.h_img : 1.0);
+	int r0 = (int)floor(0.5 * H - band), r1 = (int)ceil(0.5 * H + band);
+	r0 = r0 < 0 ? 0 : r0;
+	r1 = r1 > H ? H : r1;
 	RayRange mid = {r0 * W, r1 * W, 0, 0};
 	RayRange out = {0, r0 * W, r1 * W, H * W};
 	const unsigned n_mid = (unsigned)((r1 - r0) * W), n_out = (unsigned)(c->n_rays - n_mid);

@@ -108,7 +108,7 @@ __device__ __forceinline__ void grid_rhs_all_f32(const GridView<S> &G, const RtT
 	                     2.f * u1 * u2, 2.f * u1 * u3, u2 * u2, 2.f * u2 * u3, u3 * u3};
 	float uGu[4];
 	lev_contract<S>(G.gam[0], st, P, uGu);
-	if (st.wt >= 0.f) {
+	if (st.wt > 0.f) {   /* wt = 0: the t1 level alone (time clamped) */
 		float y2[4];
 		lev_contract<S>(G.gam[1], st, P, y2);
 #pragma unroll
@@ -123,7 +123,7 @@ __device__ __forceinline__ void grid_rhs_all_f32(const GridView<S> &G, const RtT
 	if constexpr (NF > 0) {
 		float kt, ne;
 		lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
-		if (st.wt >= 0.f) {
+		if (st.wt > 0.f) {
 			float kt2, ne2;
 			lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
 			kt = kt + st.wt * (kt2 - kt);

@@ -42,7 +42,9 @@ __device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
                                            const Stencil<double> &st, double out[NC])
 {
 	sample_lev_f64<NC, NCOMP, S>(lev[0], base, st, out);
-	if (st.wt >= 0.0) {
+	/* wt = 0 (ray time clamped to t1) reads only the t1 level: for finite
+	 * data y1 + 0 (y2 - y1) = y1, so skipping the second level is exact */
+	if (st.wt > 0.0) {
 		double y2[NC];
 		sample_lev_f64<NC, NCOMP, S>(lev[1], base, st, y2);
 #pragma unroll
