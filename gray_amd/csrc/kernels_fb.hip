@@ -26,6 +26,10 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 	unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= ctr->n_fb)
 		return;
+	/* These few waves share SIMDs with the next pool pass and their
+	 * longest ray is the render's critical path: raise their issue
+	 * priority so the pool waves fill the gaps of their dependency chain. */
+	__builtin_amdgcn_s_setprio(3);
 	Fallback f = fb[e];
 	size_t g = (size_t)f.g;
 	double s[ND];
