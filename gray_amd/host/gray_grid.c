@@ -4,6 +4,10 @@
  * sim-org/gray.c:30-59 and :147-306, over a resident device spacetime
  * (grayhip_spacetime) and a gridded render (grayhip_ctx).
  *
+ * The next level of a sliding window is read from the file on a host thread
+ * while the GPU integrates the interval before the slide (the reference
+ * reads it afterwards, gray.c:271-289).
+ *
  * Decisions for behaviour the reference leaves undefined:
  *   - rays start at q^t = 0 while the grid time is t_init + q^t, and the
  *     time weight is clamped, so only t_init must lie inside the data
@@ -16,6 +20,7 @@
 #include "gray_grid.h"
 #include "../../include/gray_spacetime.h"
 
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -38,6 +43,14 @@ struct gray_grid {
 	int frozen;
 	void *gam, *met;     /* host staging of one level */
 	void *rho;           /* and its density (transfer only) */
+	/* prefetch: the level the window needs next is read from the file on a
+	 * host thread while the GPU integrates the current interval */
+	pthread_t pf_thread;
+	int pf_active;       /* a thread is running or has finished unjoined */
+	size_t pf_level;
+	void *pf_gam, *pf_met, *pf_rho;
+	int pf_rc;
+	char pf_err[256];
 	char err[512];
 };
 
@@ -78,9 +91,14 @@ void gray_grid_destroy(gray_grid *g)
 		grayhip_spacetime_destroy(g->st);
 	free(g->names);
 	free(g->times);
+	if (g->pf_active)
+		pthread_join(g->pf_thread, NULL);
 	free(g->gam);
 	free(g->met);
 	free(g->rho);
+	free(g->pf_gam);
+	free(g->pf_met);
+	free(g->pf_rho);
 	free(g);
 }
 
@@ -96,16 +114,60 @@ static size_t find_snapshot(const gray_grid *g, double t)
 	return (size_t)(g->n_times - 2);
 }
 
-static int load_level(gray_grid *g, size_t level, int slot)
+static void *prefetch_main(void *arg)
 {
-	if (gray_h5_read_level(g->path, g->names[level], g->n, g->gam, g->met, g->rho, g->dtype))
-		return gfail(g, "%s", gray_h5_error());
-	if (grayhip_spacetime_upload(g->st, slot, g->times[level], g->gam, g->met))
+	gray_grid *g = arg;
+	g->pf_rc = gray_h5_read_level(g->path, g->names[g->pf_level], g->n, g->pf_gam, g->pf_met,
+	                              g->pf_rho, g->dtype);
+	if (g->pf_rc)
+		snprintf(g->pf_err, sizeof(g->pf_err), "%s", gray_h5_error());
+	return NULL;
+}
+
+/* Start reading `level` into the prefetch buffers (no-op if one is in
+ * flight).  The main thread makes no HDF5 call until it is joined. */
+static void prefetch_start(gray_grid *g, size_t level)
+{
+	if (g->pf_active || level >= (size_t)g->n_times)
+		return;
+	const size_t np = (size_t)g->n[0] * g->n[1] * g->n[2], sz = (size_t)g->dtype;
+	if (!g->pf_gam && !(g->pf_gam = malloc(np * 40 * sz)))
+		return;
+	if (!g->pf_met && !(g->pf_met = malloc(np * 10 * sz)))
+		return;
+	if (g->rho && !g->pf_rho && !(g->pf_rho = malloc(np * sz)))
+		return;
+	g->pf_level = level;
+	g->pf_rc = 0;
+	if (pthread_create(&g->pf_thread, NULL, prefetch_main, g) == 0)
+		g->pf_active = 1;
+}
+
+static int upload(gray_grid *g, size_t level, int slot, const void *gam, const void *met,
+                  const void *rho)
+{
+	if (grayhip_spacetime_upload(g->st, slot, g->times[level], gam, met))
 		return gfail(g, "upload of level %s: %s", g->names[level], grayhip_spacetime_error(g->st));
-	if (g->rho && grayhip_spacetime_upload_rho(g->st, slot, g->rho))
+	if (rho && grayhip_spacetime_upload_rho(g->st, slot, rho))
 		return gfail(g, "density of level %s: %s", g->names[level],
 		             grayhip_spacetime_error(g->st));
 	return 0;
+}
+
+static int load_level(gray_grid *g, size_t level, int slot)
+{
+	if (g->pf_active) {
+		pthread_join(g->pf_thread, NULL);
+		g->pf_active = 0;
+		if (g->pf_level == level) {
+			if (g->pf_rc)
+				return gfail(g, "%s", g->pf_err);
+			return upload(g, level, slot, g->pf_gam, g->pf_met, g->rho ? g->pf_rho : NULL);
+		}
+	}
+	if (gray_h5_read_level(g->path, g->names[level], g->n, g->gam, g->met, g->rho, g->dtype))
+		return gfail(g, "%s", gray_h5_error());
+	return upload(g, level, slot, g->gam, g->met, g->rho);
 }
 
 int gray_grid_load(gray_grid *g, const char *path, double t_init, double dt_dump, size_t i_init,
@@ -213,12 +275,21 @@ int gray_grid_evolve(gray_grid *g, double t, double target, size_t n_sub, double
 {
 	if (!g->ctx)
 		return gfail(g, "spacetime not loaded");
+	const double t1 = g->times[g->snap];
+	const double t2 = g->n_times > 1 ? g->times[g->snap + 1] : t1;
+	const int slide = !g->frozen && (target < t1 || target > t2);
+	if (slide) {
+		/* read the level the slide below will need while the GPU runs */
+		size_t s = find_snapshot(g, target);
+		if (s == g->snap + 1)
+			prefetch_start(g, s + 1);
+		else if (s != g->snap)
+			prefetch_start(g, s);
+	}
 	if (grayhip_evolve(g->ctx, target - t, n_sub, ns))
 		return gfail(g, "%s", grayhip_last_error(g->ctx));
 	/* gray.c:259-293: slide the window when target leaves it */
-	const double t1 = g->times[g->snap];
-	const double t2 = g->n_times > 1 ? g->times[g->snap + 1] : t1;
-	if (!g->frozen && (target < t1 || target > t2)) {
+	if (slide) {
 		size_t old = g->snap;
 		size_t s = find_snapshot(g, target);
 		if (s == old)

@@ -155,3 +155,52 @@ def test_spacetime_errors():
         with pytest.raises(G.GrayHipError):   # no density uploaded
             r.icond(1.0, 1.0, 0.5, 45.0, 0.0)
     st.close()
+
+
+def test_launcher_dyst_sliding_window_matches_oracle(tmp_path):
+    """Five levels at t = -80..0 and three intervals of -32 from t_init = 0:
+    the window slides down by one level (shift + the prefetched level), then
+    jumps (both levels reloaded), then clamps at the first level
+    (sim-org/gray.c:259-293).  The oracle replays the same windows."""
+    exe = os.path.join(os.path.dirname(J.LIB_PATH), "gray-mkgrid")
+    f = str(tmp_path / "bks5.h5")
+    r = subprocess.run([exe, f"out={f}", "n=33", "a=0.7", "v=0.3", "axis=x", "dtype=8",
+                        "times=-80.0,-60.0,-40.0,-20.0,0.0"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([J.LAUNCHER, "gray", "coor=dyst", f"data={f}", "W=8", "H=8", "w=32",
+                        "h=32", "r=35", "i=45", "p=8", "N=3", "t0=0",
+                        f"out={tmp_path}/%04d.raw"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "slow light" in r.stdout
+    names, times = S.h5_list_times(f)
+    order = np.argsort(times)
+    names, times = [names[k] for k in order], times[order]
+    lv = [S.h5_read_level(f, nm, (33, 33, 33)) for nm in names]
+
+    def find(t):
+        if t <= times[0]:
+            return 0
+        for l in range(len(times) - 1):
+            if times[l] <= t <= times[l + 1]:
+                return l
+        return len(times) - 2
+
+    def window(s):
+        return O.GridSpacetime((33,) * 3, (-40.0,) * 3, (40.0,) * 3, [lv[s][0], lv[s + 1][0]],
+                               [lv[s][1], lv[s + 1][1]], times=(times[s], times[s + 1]))
+
+    snap = find(0.0)
+    data, info = window(snap).icond_all(8, 8, 32.0, 32.0, 35.0, 45.0, 0.0)
+    steps = np.zeros(64, dtype=np.int64)
+    seen = [snap]
+    for i in range(1, 4):
+        target = -32.0 * i
+        G0 = window(snap)
+        G0.evolve_all(data, info, -32.0, 1024, steps)
+        if target < times[snap] or target > times[snap + 1]:
+            snap = find(target)
+            seen.append(snap)
+        states, diag = J.load_raw(f"{tmp_path}/{i:04d}.raw")
+        assert np.array_equal(states.reshape(-1, 8), data), i
+        assert np.array_equal(diag.ravel(), info), i
+    assert seen == [3, 2, 0, 0]
