@@ -306,3 +306,63 @@ int gray_axis_map(const double *x, int n)
 	}
 	return uni ? 0 : (fish ? 1 : -1);
 }
+
+/* The rewrite's ray dump (sim/gray.c:170-176): a new file (H5F_ACC_EXCL,
+ * an existing file is an error) with one dataset /rays of the host array
+ * palloc(real, n_width, n_height, 8) (sim/gray.c:117), i.e. dims
+ * {W, H, nd} with the last index fastest.  `data` holds the states in dump
+ * order [H][W][nd] (ray g = gi + gj W); element [gi][gj][c] of the file is
+ * data[(gj W + gi) nd + c]. */
+int gray_h5_write_rays(const char *path, int w, int h, int nd, const double *data, int dtype)
+{
+	const size_t n = (size_t)w * h * nd;
+	double *buf = malloc(n * sizeof(double));
+	if (!buf)
+		return h5fail("out of memory");
+	for (size_t gi = 0; gi < (size_t)w; ++gi)
+		for (size_t gj = 0; gj < (size_t)h; ++gj)
+			memcpy(buf + (gi * h + gj) * nd, data + (gj * w + gi) * nd, nd * sizeof(double));
+	hid_t f = H5Fcreate(path, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT);
+	if (f < 0) {
+		free(buf);
+		return h5fail("cannot create %s (it must not exist)", path);
+	}
+	hsize_t dims[3] = {(hsize_t)w, (hsize_t)h, (hsize_t)nd};
+	hid_t sp = H5Screate_simple(3, dims, NULL);
+	hid_t ds = sp < 0 ? -1 : H5Dcreate2(f, "rays", file_type(dtype), sp, H5P_DEFAULT,
+	                                      H5P_DEFAULT, H5P_DEFAULT);
+	int rc = ds < 0 || H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) < 0;
+	if (ds >= 0)
+		H5Dclose(ds);
+	if (sp >= 0)
+		H5Sclose(sp);
+	H5Fclose(f);
+	free(buf);
+	return rc ? h5fail("writing /rays of %s failed", path) : 0;
+}
+
+int gray_h5_read_rays(const char *path, int dims[3], double *out)
+{
+	hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+	if (f < 0)
+		return h5fail("cannot open %s", path);
+	hid_t ds = H5Dopen2(f, "rays", H5P_DEFAULT);
+	int rc = -1;
+	if (ds >= 0) {
+		hid_t sp = H5Dget_space(ds);
+		hsize_t d[3] = {0, 0, 0};
+		if (sp >= 0 && H5Sget_simple_extent_ndims(sp) == 3) {
+			H5Sget_simple_extent_dims(sp, d, NULL);
+			for (int k = 0; k < 3; ++k)
+				dims[k] = (int)d[k];
+			rc = out ? (H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0
+			            ? -1 : 0)
+			         : 0;
+		}
+		if (sp >= 0)
+			H5Sclose(sp);
+		H5Dclose(ds);
+	}
+	H5Fclose(f);
+	return rc ? h5fail("%s has no 3-d /rays dataset", path) : 0;
+}

@@ -10,6 +10,7 @@
  */
 #define _GNU_SOURCE
 #include "../../include/gray_job.h"
+#include "../../include/gray_spacetime.h"
 #include "../../include/grayhip.h"
 #include "gray_grid.h"
 #include "gray_opts.h"
@@ -32,10 +33,16 @@ struct gray {
 
 #define EGO ((struct gray *)ego)
 
+/* The rewrite's job surface (sim/gray.c): selected by spacetime=..., its
+ * planner switch (sim/gray.c:49-58), with initcond=infcam and ray files. */
+static int rewrite(const struct gray_options *o)
+{
+	return o->spacetime && *o->spacetime;
+}
+
 static int analytic(const struct gray_options *o)
 {
-	return (o->coordinates && !strcmp(o->coordinates, "KS")) ||
-	       (o->spacetime && !strcmp(o->spacetime, "Kerr"));
+	return (o->coordinates && !strcmp(o->coordinates, "KS")) || rewrite(o);
 }
 
 static int _conf(void *ego, const char *arg)
@@ -84,6 +91,30 @@ static int _init(void *ego)
 	if (!o->scheme || strcmp(o->scheme, "RK4")) {
 		fprintf(stderr, "GRay2: unknown integration scheme \"%s\" (RK4 only)\n", o->scheme);
 		return EXIT_FAILURE;
+	}
+	if (rewrite(o)) {
+		/* sim/gray.c:49-70: Kerr and infcam are the only modules; the rays
+		 * are real[8] (sim/gray.c:116-117), so no transfer channels */
+		if (strcmp(o->spacetime, "Kerr")) {
+			fprintf(stderr, "GRay2: Unknown spacetime configuration \"%s\"\n", o->spacetime);
+			return EXIT_FAILURE;
+		}
+		if (!o->initcond || strcmp(o->initcond, "infcam")) {
+			fprintf(stderr, "GRay2: Unknown initial conditions for rays \"%s\"\n",
+			        o->initcond ? o->initcond : "");
+			return EXIT_FAILURE;
+		}
+		if (o->coordinates && strcmp(o->coordinates, "pxcenter") &&
+		    strcmp(o->coordinates, "dyst") && strcmp(o->coordinates, "KS")) {
+			fprintf(stderr, "GRay2: infcam coordinates \"%s\" (pxcenter only)\n",
+			        o->coordinates);
+			return EXIT_FAILURE;
+		}
+		if (EGO->n_freq) {
+			fprintf(stderr, "GRay2: the ray files of spacetime=Kerr hold 8 reals per ray "
+			                "(no nu)\n");
+			return EXIT_FAILURE;
+		}
 	}
 	if (!o->morder || (strcmp(o->morder, "AoS") && strcmp(o->morder, "SoA"))) {
 		fprintf(stderr, "GRay2: unknown memory order \"%s\" (AoS or SoA)\n", o->morder);
@@ -141,7 +172,7 @@ static int _init(void *ego)
 			return EXIT_FAILURE;
 		}
 	}
-	EGO->data = malloc(n_rays * n_data * sz);
+	EGO->data = malloc(n_rays * n_data * (rewrite(o) ? sizeof(double) : sz));
 	EGO->info = malloc(n_rays * sz);
 	EGO->tmp = malloc(n_rays * n_data * sz);
 	if (!EGO->data || !EGO->info || !EGO->tmp) {
@@ -184,6 +215,45 @@ static int dump(void *ego, size_t i)
 	return 0;
 }
 
+/* sim/gray.c:153-185: n_dump intervals ending at dt_dump * (i + 1), each
+ * followed by rayfile % (i + 1) holding /rays.  The rewrite leaves the
+ * integration and the infcam initial conditions as TODOs
+ * (sim/gray.c:168, sim/gray/infcam.c:40); here they are the pixel-centre
+ * camera of sim-org/driver.cl:36-67 and the analytic Kerr integration. */
+static int exec_rewrite(void *ego)
+{
+	struct gray_options *o = &EGO->o;
+	const int w = (int)o->w_rays, h = (int)o->h_rays;
+	double t = o->t_init;
+	size_t i = o->i_init;
+	if (grayhip_icond(EGO->ctx, o->w_img, o->h_img, o->r_obs, o->i_obs, o->j_obs)) {
+		fprintf(stderr, "GRay2: icond: %s\n", grayhip_last_error(EGO->ctx));
+		return EXIT_FAILURE;
+	}
+	while (i < o->n_dump) {
+		const size_t next = i + 1;
+		const double target = o->dt_dump * (double)next;
+		double ns = 0.0;
+		char name[4096];
+		printf("%zu: %4.1f -> %4.1f", next, t, target);
+		fflush(stdout);
+		if (grayhip_evolve(EGO->ctx, target - t, o->n_sub, &ns) ||
+		    grayhip_read_f64(EGO->ctx, EGO->data, NULL)) {
+			fprintf(stderr, "\nGRay2: evolve: %s\n", grayhip_last_error(EGO->ctx));
+			return EXIT_FAILURE;
+		}
+		snprintf(name, sizeof(name), o->rayfile, (int)next);
+		if (gray_h5_write_rays(name, w, h, 8, EGO->data, (int)o->precision)) {
+			fprintf(stderr, "\nGRay2: %s\n", gray_h5_error());
+			return EXIT_FAILURE;
+		}
+		printf(": DONE\n");
+		i = next;
+		t = target;
+	}
+	return EXIT_SUCCESS;
+}
+
 static int _exec(void *ego)
 {
 	struct gray_options *o = &EGO->o;
@@ -196,6 +266,8 @@ static int _exec(void *ego)
 		fprintf(stderr, "GRay2: exec before init\n");
 		return EXIT_FAILURE;
 	}
+	if (rewrite(o))
+		return exec_rewrite(ego);
 	if (EGO->grid) {
 		/* gray.c:164-241: open the file, list times, load the bracketing
 		 * snapshots (or one, frozen) */

@@ -70,9 +70,11 @@ static const opt_row rows[] = {
 	{"m_bh:M", T_REAL, OFF(m_bh), "4e6", "Kerr.opts:1", "Black-hole mass in solar masses"},
 	{"a_spin:a", T_REAL, OFF(a_spin), "0", "Kerr.opts:2", "Dimensionless black-hole spin"},
 	/* sim/gray.opts:5-14, sim/infcam.opts:9 (rewrite) */
-	{"spacetime:st", T_STR, OFF(spacetime), "", "gray.opts:5", "Kerr selects the analytic spacetime"},
+	{"spacetime:st", T_STR, OFF(spacetime), "", "gray.opts:5",
+	 "Kerr selects the rewrite's job: analytic Kerr, infcam rays, HDF5 ray files"},
 	{"initcond:ic", T_STR, OFF(initcond), "infcam", "gray.opts:7", "Initial conditions (infcam)"},
-	{"rayfile:ray", T_STR, OFF(rayfile), "ray%04d.h5", "gray.opts:14", "Ray dump name (rewrite; unused)"},
+	{"rayfile:ray", T_STR, OFF(rayfile), "ray%04d.h5", "gray.opts:14",
+	 "Ray file name format of spacetime=Kerr (/rays, dims W x H x 8)"},
 	/* build additions */
 	{"escape:term", T_STR, OFF(escape), "outward", "build",
 	 "Termination: outward (r<1.01 r_H, or r>r_esc moving outward) or ref (as phys.cl)"},

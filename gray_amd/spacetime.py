@@ -9,7 +9,8 @@ from . import job as _job
 
 EXPORTS = ("gray_bks_setup", "gray_bks_point", "gray_bks_sample", "gray_axis_uniform",
            "gray_axis_fisheye", "gray_h5_error", "gray_h5_create", "gray_h5_write_level",
-           "gray_h5_list_times", "gray_h5_read_axes", "gray_h5_read_level", "gray_axis_map")
+           "gray_h5_list_times", "gray_h5_read_axes", "gray_h5_read_level", "gray_axis_map",
+           "gray_h5_write_rays", "gray_h5_read_rays")
 
 
 class BKS(C.Structure):
@@ -45,6 +46,8 @@ def lib():
         L.gray_h5_read_level.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int), vp, vp, vp,
                                          C.c_int]
         L.gray_axis_map.argtypes = [dp, C.c_int]
+        L.gray_h5_write_rays.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, C.c_int]
+        L.gray_h5_read_rays.argtypes = [C.c_char_p, C.POINTER(C.c_int), vp]
         _set = True
     return L
 
@@ -142,3 +145,19 @@ def h5_read_level(path, name, n, dtype=8, rho=False):
     _h5(lib().gray_h5_read_level(path.encode(), name.encode(), nn, _vp(gam), _vp(met), _vp(r),
                                  dtype))
     return (gam, met, r) if rho else (gam, met)
+
+
+def h5_write_rays(path, states, dtype=8):
+    """states: [H, W, nd] in dump order; file /rays is [W, H, nd]."""
+    s = np.ascontiguousarray(states, dtype=np.float64)
+    h, w, nd = s.shape
+    _h5(lib().gray_h5_write_rays(path.encode(), w, h, nd, _vp(s), dtype))
+
+
+def h5_read_rays(path):
+    """/rays of a ray file as FP64, in file order [W, H, nd]."""
+    dims = (C.c_int * 3)()
+    _h5(lib().gray_h5_read_rays(path.encode(), dims, None))
+    out = np.empty(tuple(dims), dtype=np.float64)
+    _h5(lib().gray_h5_read_rays(path.encode(), dims, _vp(out)))
+    return out

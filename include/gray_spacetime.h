@@ -49,6 +49,13 @@ int gray_h5_create(const char *path, int nx, int ny, int nz, const double *xs,
 /* Append one time level named `name` from interleaved FP64 arrays. */
 int gray_h5_write_level(const char *path, const char *name, int nx, int ny, int nz,
                         const double *gam, const double *met, const double *rho, int dtype);
+/* The rewrite's ray dump (replaces sim/gray.c:170-176): create `path`
+ * (error if it exists) with dataset /rays of dims {W, H, nd} from states in
+ * dump order [H][W][nd], stored as dtype (4 or 8) bytes per real. */
+int gray_h5_write_rays(const char *path, int w, int h, int nd, const double *data, int dtype);
+/* Read /rays: dims into dims[3]; values (if out is not NULL) as FP64 in file
+ * order [W][H][nd]. */
+int gray_h5_read_rays(const char *path, int dims[3], double *out);
 /* Time levels: names (sorted by time) into names[max][64], times[max];
  * returns the count or -1 (fewer than 2 groups is an error, io.c:182-185). */
 int gray_h5_list_times(const char *path, char (*names)[64], double *times, int max);

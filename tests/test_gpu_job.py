@@ -46,3 +46,23 @@ def test_launcher_soa_and_single_precision(tmp_path):
     data, _ = O.icond_all(p, 8, 4, 64.0, 64.0, 1024.0, 60.0, 0.0)
     np.testing.assert_array_equal(soa.T, data.astype(np.float32))
     assert os.path.exists(f"{tmp_path}/0001.raw")
+
+
+def test_rewrite_surface_ray_files_match_oracle(tmp_path):
+    """spacetime=Kerr (sim/gray.c): infcam rays, n_dump intervals ending at
+    dt_dump * (i + 1), ray%04d.h5:/rays of dims W x H x 8 after each."""
+    from gray_amd import spacetime as S
+    r = subprocess.run([J.LAUNCHER, "gray", "st=Kerr", "ic=infcam", "coor=pxcenter", "a=0.9375",
+                        "n_width=12", "n_height=8", "w=48", "h=32", "r=1000", "i=60", "p=8",
+                        "N=2", f"ray={tmp_path}/ray%04d.h5"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "2: -32.0 -> -64.0" in r.stdout
+    assert not os.path.exists(f"{tmp_path}/ray0000.h5")
+    p = O.make_params(0.9375)
+    data, info = O.icond_all(p, 12, 8, 48.0, 32.0, 1000.0, 60.0, 0.0)
+    for i in (1, 2):
+        O.evolve_all(p, data, info, -32.0, 1024)
+        rays = S.h5_read_rays(f"{tmp_path}/ray{i:04d}.h5")
+        assert rays.shape == (12, 8, 8)
+        assert np.array_equal(rays, data.reshape(8, 12, 8).transpose(1, 0, 2)), i

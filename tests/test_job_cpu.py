@@ -108,3 +108,33 @@ def test_init_fails_cleanly_on_bad_setup():
         assert j.init() != 0
     with J.Job() as j:
         assert j.exec() != 0  # exec before init
+
+
+def test_ray_file_layout_and_exclusive_create(tmp_path):
+    """sim/gray.c:117,176: /rays holds palloc(real, W, H, 8), last index
+    fastest; the file is created with H5F_ACC_EXCL."""
+    from gray_amd import spacetime as S
+    rng = np.random.default_rng(4)
+    states = rng.normal(size=(3, 5, 8))          # [H, W, 8] dump order
+    p = str(tmp_path / "ray0001.h5")
+    S.h5_write_rays(p, states)
+    got = S.h5_read_rays(p)
+    assert got.shape == (5, 3, 8)
+    assert np.array_equal(got, states.transpose(1, 0, 2))
+    with pytest.raises(S.H5Error):
+        S.h5_write_rays(p, states)               # exists
+    p4 = str(tmp_path / "ray0002.h5")
+    S.h5_write_rays(p4, states, dtype=4)
+    assert np.array_equal(S.h5_read_rays(p4), states.transpose(1, 0, 2).astype(np.float32))
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["st=Schwarzschild"], "Unknown spacetime configuration"),
+    (["st=Kerr", "ic=plane"], "Unknown initial conditions"),
+    (["st=Kerr", "nu=1e11"], "8 reals per ray"),
+    (["st=Kerr", "coor=polar"], "pxcenter"),
+])
+def test_rewrite_surface_rejects(args, msg):
+    import subprocess
+    r = subprocess.run([J.LAUNCHER, "gray", "W=4", "H=4"] + args, capture_output=True, text=True)
+    assert r.returncode != 0 and msg in r.stderr, r.stderr
