@@ -479,6 +479,22 @@ extern "C" int grayhip_read(grayhip_ctx *c, void *data, void *info)
 	return GRAYHIP_OK;
 }
 
+extern "C" int grayhip_read_bl(grayhip_ctx *c, double r_match, double *out)
+{
+	if (!c || !out)
+		return fail(c, GRAYHIP_EINVAL, "null argument");
+	if (c->st)
+		return fail(c, GRAYHIP_EINVAL, "the Boyer-Lindquist view needs the analytic spacetime");
+	if (!(r_match > c->p.r_h))
+		return fail(c, GRAYHIP_EINVAL, "r_match must lie outside the horizon");
+	HCK(c, hipSetDevice(c->cfg.device));
+	HCK(c, launch_to_bl(c->p.a, r_match, c->d_data, (double *)c->d_tmp, c->n_rays, c->stream));
+	HCK(c, hipMemcpyAsync(out, c->d_tmp, c->n_rays * 8 * sizeof(double), hipMemcpyDeviceToHost,
+	                      c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	return GRAYHIP_OK;
+}
+
 extern "C" int grayhip_read_f64(grayhip_ctx *c, double *data, double *info)
 {
 	if (!c)

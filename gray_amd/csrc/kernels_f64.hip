@@ -102,7 +102,71 @@ __global__ __launch_bounds__(256) void k_info_f64(RayParams p, const double *__r
 	info[g] = getuu(p.a, q, u);
 }
 
+/* Boyer-Lindquist view of the final states (sim-org/KS.cl:63-149) in the
+ * spheroidal convention of the metric, x + i y = (r + i a) e^{i phi}
+ * sin(theta), and with u^r in the t and phi corrections (the derivative of
+ * the position map of KS.cl:79-88).  Out: AoS [g][8] = (t, r, theta, phi,
+ * u^t, u^r, u^theta, u^phi) in BL; t matched at r_match, phi at infinity.
+ * Same expressions and order as the oracle; acos, atan2 and log are not
+ * correctly rounded, so agreement is to ~1e-15 relative, not bitwise. */
+__global__ __launch_bounds__(256) void k_to_bl(double a, double r_match,
+                                               const double *__restrict__ data,
+                                               double *__restrict__ out, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	const double q[4] = {data[g], data[n_rays + g], data[2 * n_rays + g], data[3 * n_rays + g]};
+	const double u[4] = {data[4 * n_rays + g], data[5 * n_rays + g], data[6 * n_rays + g],
+	                     data[7 * n_rays + g]};
+	const double aa = a * a;
+	const double RR = q[1] * q[1] + q[2] * q[2];
+	const double zz = q[3] * q[3];
+	/* position (og_getsphks, og_getbl) */
+	const double kk0 = 0.5 * ((RR + zz) - aa);
+	const double r = sqrt(sqrt(kk0 * kk0 + aa * zz) + kk0);
+	const double h = sqrt(1.0 - aa), rp = 1.0 + h, rm = 1.0 - h;
+	double *o = out + (size_t)8 * g;
+	o[0] = q[0] + (rm * log((r - rm) / (r_match - rm)) - rp * log((r - rp) / (r_match - rp))) / h;
+	o[1] = r;
+	o[2] = acos(q[3] / r);
+	o[3] = atan2(q[2] * r - q[1] * a, q[1] * r + q[2] * a) + ((0.5 * a) * log((r - rm) / (r - rp))) / h;
+	/* velocity (og_getsphksu, og_getblu) */
+	const double kk = 0.5 * ((RR + zz) - aa);
+	const double dd = sqrt(kk * kk + aa * zz);
+	const double rr = dd + kk;
+	const double rs = sqrt(rr);
+	const double d = (2.0 * rs) * dd;
+	const double J1[4] = {0.0, (rr * q[1]) / d, (rr * q[2]) / d, ((rr + aa) * q[3]) / d};
+	const double sz = sqrt(rr - zz);
+	const double J2[4] = {((q[3] / rs) * J1[0]) / sz, ((q[3] / rs) * J1[1]) / sz,
+	                      ((q[3] / rs) * J1[2]) / sz, ((q[3] / rs) * J1[3] - 1.0) / sz};
+	const double c = a / (rr + aa);
+	const double J3[4] = {c * J1[0], c * J1[1] + (-q[2]) / RR, c * J1[2] + q[1] / RR, c * J1[3]};
+	const double ur = ((J1[0] * u[0] + J1[1] * u[1]) + J1[2] * u[2]) + J1[3] * u[3];
+	const double ut = ((J2[0] * u[0] + J2[1] * u[1]) + J2[2] * u[2]) + J2[3] * u[3];
+	const double up = ((J3[0] * u[0] + J3[1] * u[1]) + J3[2] * u[2]) + J3[3] * u[3];
+	const double kk2 = 0.5 * (((q[1] * q[1] + q[2] * q[2]) + zz) - aa);
+	const double rr2 = sqrt(kk2 * kk2 + aa * zz) + kk2;
+	const double r2 = sqrt(rr2);
+	const double D = (rr2 - 2.0 * r2) + aa;
+	o[4] = u[0] - (ur * (2.0 * r2)) / D;
+	o[5] = ur;
+	o[6] = ut;
+	o[7] = up - (ur * a) / D;
+}
+
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_to_bl(double a, double r_match, const double *data, double *out, size_t n_rays,
+                        hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(k_to_bl, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, a, r_match, data,
+	                   out, n_rays);
+	return hipGetLastError();
+}
 
 hipError_t launch_icond(const RayParams &p, double *data, double *info, int64_t *steps,
                         size_t n_rays, hipStream_t st)

@@ -42,6 +42,8 @@ hipError_t launch_aos_to_soa(const void *aos, double *soa, int real_size, size_t
 hipError_t launch_cast(const double *in, void *out, int real_size, size_t n, hipStream_t st);
 hipError_t launch_info_f64(const RayParams &p, const double *data, double *info, size_t n_rays,
                            hipStream_t st);
+hipError_t launch_to_bl(double a, double r_match, const double *data, double *out, size_t n_rays,
+                        hipStream_t st);
 
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,

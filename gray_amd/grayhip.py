@@ -21,7 +21,7 @@ EXPORTS = (
     "grayhip_create", "grayhip_destroy", "grayhip_icond", "grayhip_evolve",
     "grayhip_render", "grayhip_read", "grayhip_read_f64", "grayhip_write",
     "grayhip_read_steps", "grayhip_stats", "grayhip_pass_steps", "grayhip_kernel_time",
-    "grayhip_n_data",
+    "grayhip_n_data", "grayhip_read_bl",
     "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
     "grayhip_spacetime_create", "grayhip_spacetime_destroy", "grayhip_spacetime_upload",
     "grayhip_spacetime_shift", "grayhip_spacetime_set_frozen", "grayhip_spacetime_bytes",
@@ -91,6 +91,7 @@ def lib():
     L.grayhip_pass_steps.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.grayhip_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.grayhip_n_data.argtypes = [vp]
+    L.grayhip_read_bl.argtypes = [vp, C.c_double, vp]
     L.grayhip_last_error.argtypes = [vp]
     L.grayhip_last_error.restype = C.c_char_p
     L.grayhip_global_error.argtypes = []
@@ -271,6 +272,12 @@ class Render:
         info = np.empty(self.n_rays, dtype=np.float64)
         self._check(lib().grayhip_read_f64(self._h, _ptr(data), _ptr(info)), "read_f64")
         return data, info
+
+    def read_bl(self, r_match):
+        """Boyer-Lindquist states [n_rays, 8]: t, r, theta, phi, u^t, u^r, u^theta, u^phi."""
+        out = np.empty((self.n_rays, 8), dtype=np.float64)
+        self._check(lib().grayhip_read_bl(self._h, r_match, _ptr(out)), "read_bl")
+        return out
 
     def write(self, data):
         t = np.float64 if self.cfg.precision == FP64 else np.float32

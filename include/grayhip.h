@@ -144,6 +144,15 @@ int  grayhip_read(grayhip_ctx *ctx, void *data, void *info);
  * both forms): data AoS [H][W][n_data] doubles, info n_rays doubles. */
 int  grayhip_read_f64(grayhip_ctx *ctx, double *data, double *info);
 
+/* Boyer-Lindquist view of the states (analytic Kerr only; replaces the
+ * getBL/getBLu diagnostics of sim-org/KS.cl:63-149, which need the
+ * undefined r_match): AoS [H][W][8] doubles (t, r, theta, phi, u^t, u^r,
+ * u^theta, u^phi) with t matched at r = r_match (> r_H) and phi at
+ * infinity.  Spheroidal convention x + i y = (r + i a) e^{i phi} sin(theta),
+ * the one in which the metric maps to BL by r-only shifts; the velocity
+ * uses u^r in the t and phi corrections.  Rays inside the horizon give NaN. */
+int  grayhip_read_bl(grayhip_ctx *ctx, double r_match, double *out);
+
 /* Upload AoS states (same layout as grayhip_read) to restart from a dump. */
 int  grayhip_write(grayhip_ctx *ctx, const void *data);
 

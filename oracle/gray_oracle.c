@@ -458,3 +458,72 @@ int64_t og_evolve_subset(const og_params *p, size_t n_rays, size_t stride,
 	}
 	return total;
 }
+
+/* KS.cl:63-71 in the spheroidal convention of the metric of
+ * KS.cl:100-108, x + i y = (r + i a) e^{i phi} sin(theta): phi =
+ * atan2(y r - x a, x r + y a).  (KS.cl:68-69 inverts the mirrored
+ * convention, r - i a, in which this metric has no r-only map to
+ * Boyer-Lindquist coordinates.) */
+void og_getsphks(double a, const double q[4], double out[4])
+{
+	const double r = sqrt(og_getrr(a, q));
+	out[0] = q[0];
+	out[1] = r;
+	out[2] = acos(q[3] / r);
+	out[3] = atan2(q[2] * r - q[1] * a, q[1] * r + q[2] * a);
+}
+
+/* KS.cl:73-89 */
+void og_getbl(double a, double r_match, const double q[4], double out[4])
+{
+	double s[4];
+	og_getsphks(a, q, s);
+	const double h = sqrt(1.0 - a * a), rp = 1.0 + h, rm = 1.0 - h, r = s[1];
+	out[0] = s[0] + (rm * log((r - rm) / (r_match - rm)) - rp * log((r - rp) / (r_match - rp))) / h;
+	out[1] = s[1];
+	out[2] = s[2];
+	out[3] = s[3] + ((0.5 * a) * log((r - rm) / (r - rp))) / h;
+}
+
+/* KS.cl:113-131: J1 = (0, rr x, rr y, (rr + aa) z) / (2 r dd) = dr,
+ * J2 = ((z / r) J1 - (0,0,0,1)) / sqrt(rr - zz) = dtheta,
+ * J3 = (a / (rr + aa)) J1 + (0, -y, x, 0) / RR = dphi of og_getsphks
+ * (KS.cl:125 has -a, the derivative of its mirrored phi). */
+void og_getsphksu(double a, const double q[4], const double u[4], double out[4])
+{
+	const double aa = a * a;
+	const double RR = q[1] * q[1] + q[2] * q[2];
+	const double zz = q[3] * q[3];
+	const double kk = 0.5 * ((RR + zz) - aa);
+	const double dd = sqrt(kk * kk + aa * zz);
+	const double rr = dd + kk;
+	const double r = sqrt(rr);
+	const double d = (2.0 * r) * dd;
+	const double J1[4] = {0.0, (rr * q[1]) / d, (rr * q[2]) / d, ((rr + aa) * q[3]) / d};
+	const double sz = sqrt(rr - zz);
+	const double J2[4] = {((q[3] / r) * J1[0]) / sz, ((q[3] / r) * J1[1]) / sz,
+	                      ((q[3] / r) * J1[2]) / sz, ((q[3] / r) * J1[3] - 1.0) / sz};
+	const double c = a / (rr + aa);
+	const double J3[4] = {c * J1[0], c * J1[1] + (-q[2]) / RR, c * J1[2] + q[1] / RR, c * J1[3]};
+	out[0] = u[0];
+	out[1] = ((J1[0] * u[0] + J1[1] * u[1]) + J1[2] * u[2]) + J1[3] * u[3];
+	out[2] = ((J2[0] * u[0] + J2[1] * u[1]) + J2[2] * u[2]) + J2[3] * u[3];
+	out[3] = ((J3[0] * u[0] + J3[1] * u[1]) + J3[2] * u[2]) + J3[3] * u[3];
+}
+
+/* KS.cl:133-149 with u^r in the t and phi corrections (see the header). */
+void og_getblu(double a, const double q[4], const double u[4], double out[4])
+{
+	double s[4];
+	og_getsphksu(a, q, u, s);
+	const double aa = a * a;
+	const double zz = q[3] * q[3];
+	const double kk = 0.5 * (((q[1] * q[1] + q[2] * q[2]) + zz) - aa);
+	const double rr = sqrt(kk * kk + aa * zz) + kk;
+	const double r = sqrt(rr);
+	const double D = (rr - 2.0 * r) + aa;
+	out[0] = s[0] - (s[1] * (2.0 * r)) / D;
+	out[1] = s[1];
+	out[2] = s[2];
+	out[3] = s[3] - (s[1] * a) / D;
+}

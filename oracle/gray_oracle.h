@@ -106,6 +106,20 @@ void    og_icond_all(const og_params *p, size_t w_rays, size_t h_rays,
 int64_t og_evolve_all(const og_params *p, size_t n_rays, double *data,
                       double *info, double dt, size_t n_sub,
                       int64_t *steps_out, int n_threads);
+/* Boyer-Lindquist view of a Kerr-Schild state (KS.cl:63-149): spherical
+ * Kerr-Schild position (t, r, theta, phi) and its BL form, with t matched
+ * at r = r_match and phi at r = infinity; the velocity in spherical KS and
+ * in BL.  Spheroidal convention x + i y = (r + i a) e^{i phi} sin(theta),
+ * the one in which the metric of KS.cl:100-108 maps to BL by the r-only
+ * shifts of KS.cl:79-88 (KS.cl:68-69,125 use the mirror image, r - i a).
+ * KS.cl:146,149 multiply u^phi and u^t where the r-derivative of the
+ * position map requires u^r; this follows the derivative:
+ * u^t_BL = u^t - 2 r u^r / Delta, u^phi_BL = u^phi - a u^r / Delta. */
+void   og_getsphks(double a, const double q[4], double out[4]);
+void   og_getbl(double a, double r_match, const double q[4], double out[4]);
+void   og_getsphksu(double a, const double q[4], const double u[4], double out[4]);
+void   og_getblu(double a, const double q[4], const double u[4], double out[4]);
+
 /* Evolve a strided subset of rays (every stride-th ray starting at 0); used
  * for bounded CPU-baseline timing.  Returns total steps. */
 int64_t og_evolve_subset(const og_params *p, size_t n_rays, size_t stride,

@@ -149,6 +149,40 @@ def getdt(p, q, u, dt):
     return lib().og_getdt(C.byref(p), _dp(_vec(q)), _dp(_vec(u)), dt)
 
 
+def _bl_set():
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    L.og_getsphks.argtypes = [C.c_double, dp, dp]
+    L.og_getbl.argtypes = [C.c_double, C.c_double, dp, dp]
+    L.og_getsphksu.argtypes = [C.c_double, dp, dp, dp]
+    L.og_getblu.argtypes = [C.c_double, dp, dp, dp]
+    return L
+
+
+def getsphks(a, q):
+    out = np.zeros(4)
+    _bl_set().og_getsphks(a, _dp(_vec(q)), _dp(out))
+    return out
+
+
+def getbl(a, r_match, q):
+    out = np.zeros(4)
+    _bl_set().og_getbl(a, r_match, _dp(_vec(q)), _dp(out))
+    return out
+
+
+def getsphksu(a, q, u):
+    out = np.zeros(4)
+    _bl_set().og_getsphksu(a, _dp(_vec(q)), _dp(_vec(u)), _dp(out))
+    return out
+
+
+def getblu(a, q, u):
+    out = np.zeros(4)
+    _bl_set().og_getblu(a, _dp(_vec(q)), _dp(_vec(u)), _dp(out))
+    return out
+
+
 def flow_disk(p, q, k):
     n = C.c_double()
     sh = C.c_double()

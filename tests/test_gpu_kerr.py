@@ -161,3 +161,23 @@ def test_errors_are_reported():
         r.icond(64.0, 64.0, 1000.0, 60.0, 0.0)
         with pytest.raises(G.GrayHipError):
             r.evolve(-32.0, 0)
+
+
+def test_boyer_lindquist_view_matches_oracle():
+    """grayhip_read_bl against the oracle's getbl/getblu on the same final
+    states (acos, atan2 and log are not correctly rounded: 1e-12)."""
+    with G.Render(24, 16, A, precision=G.FP64) as r:
+        r.icond(48.0, 32.0, 1000.0, 60.0, 0.0)
+        r.render(-32.0, 1024, 20)
+        d, _ = r.read_f64()
+        bl = r.read_bl(50.0)
+    want = np.array([np.concatenate([O.getbl(A, 50.0, s[:4]), O.getblu(A, s[:4], s[4:8])])
+                     for s in d])
+    fin = np.isfinite(want)
+    assert np.array_equal(fin, np.isfinite(bl))
+    assert fin.all(axis=1).sum() > 300
+    np.testing.assert_allclose(bl[fin], want[fin], rtol=1e-12, atol=1e-12)
+    with pytest.raises(G.GrayHipError):
+        with G.Render(4, 4, A, precision=G.FP64) as r2:
+            r2.icond(8.0, 8.0, 1000.0, 60.0, 0.0)
+            r2.read_bl(1.0)            # r_match inside the horizon
