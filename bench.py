@@ -90,7 +90,7 @@ def headline(G, args, out):
         r, wall, ns = render(G, G.FP32)
         walls.append(wall)
         st, nfb = r.stats()
-        ps, _ = r.pass_steps()
+        ps, fbs = r.pass_steps()
         kn, nl = r.kernel_time()
         total_steps += st
         kern_ns += kn
@@ -99,6 +99,7 @@ def headline(G, args, out):
             d32, _ = r.read_f64()
             img32 = d32[:, 8:12].copy()
             out["fallback_rays"] = nfb
+            out["fallback_ray_steps"] = fbs
         r.close()
     t_all = time.perf_counter() - t_all
     value = total_steps / t_all
