@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_icond(RayParams p, double *__restrict__
 
 /* Plain FP64 form: one lane per ray, one interval per launch. */
 template <int NF>
-__global__ __launch_bounds__(256) void k_evolve_f64(RayParams p, StepRule sr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 4 ? 2 : 1))) void k_evolve_f64(RayParams p, StepRule sr,
                                                     double *__restrict__ data,
                                                     double *__restrict__ info,
                                                     int64_t *__restrict__ steps,

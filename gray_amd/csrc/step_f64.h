@@ -38,15 +38,17 @@ __device__ __forceinline__ void rk4_f64(const P &p, double *s, double dt)
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
 	rhs_f64<NF, P>(p, x, k);
+	/* k1 + 2 (k2 + k3) is formed before the last stage (same value), so
+	 * k1 and k2 + k3 are not both live across it */
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
-		k23[i] = k23[i] + k[i];
+		k1[i] = k1[i] + 2.0 * (k23[i] + k[i]);
 		x[i] = s[i] + dt * k[i];
 	}
 	rhs_f64<NF, P>(p, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
-		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k23[i]) + k[i])) / 6.0;
+		s[i] = s[i] + (dt * (k1[i] + k[i])) / 6.0;
 }
 
 /* Sub-steps of one interval from unit h0 (sim-org/driver.cl:92-99).
