@@ -1,0 +1,73 @@
+"""Edge cases of the analytic-Kerr path on the GPU against the oracle:
+degenerate and ragged image shapes, spin at 0, negative and near-extremal,
+pole-on, edge-on and south-pole cameras, the maximum of 8 channels, and a
+zoomed plane entirely inside the shadow.
+
+Bars as in test_gpu_kerr.py: the plain FP64 form equals the oracle bit for
+bit on geodesic state and step counts, transfer channels to 1e-10; the FP32
+form stays within image relative L1 1e-4 of the oracle (checked where the
+image has emission)."""
+import numpy as np
+import pytest
+
+from gray_amd import grayhip as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NUS8 = (1.0, 1.5, 2.0, 2.67, 3.0, 4.0, 6.0, 8.0)
+
+
+def both(a, w, h, w_img, h_img, i_obs, j_obs=0.0, nu=(), n_dump=128, precision=G.FP64):
+    p = O.make_params(a, nu=nu)
+    d0, i0, s0 = O.render(p, w, h, w_img, h_img, 1000.0, i_obs, j_obs, n_dump=n_dump)
+    with G.Render(w, h, a, precision=precision, nu=nu) as r:
+        r.icond(w_img, h_img, 1000.0, i_obs, j_obs)
+        r.render(-32.0, 1024, n_dump)
+        d1, i1 = r.read_f64()
+        s1 = r.steps()
+    return (d0, i0, s0), (d1, i1, s1)
+
+
+def check_fp64(ref, got, nf):
+    (d0, i0, s0), (d1, i1, s1) = ref, got
+    assert np.array_equal(s0, s1), f"{(s0 != s1).sum()} rays differ in step count"
+    assert np.array_equal(d0[:, :8], d1[:, :8])
+    assert np.array_equal(i0, i1)
+    if nf:
+        rt0, rt1 = d0[:, 8:], d1[:, 8:]
+        assert np.all(np.abs(rt1 - rt0) <= 1e-10 * np.abs(rt0)), np.abs(rt1 - rt0).max()
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (3, 7), (1, 33)])
+def test_shapes(w, h):
+    check_fp64(*both(0.9375, w, h, 24.0, 16.0, 60.0, nu=(1.0, 4.0)), 2)
+
+
+@pytest.mark.parametrize("a", [0.0, -0.7, 0.999])
+def test_spins(a):
+    check_fp64(*both(a, 12, 10, 24.0, 20.0, 60.0, j_obs=20.0, nu=(2.67,)), 1)
+
+
+@pytest.mark.parametrize("i_obs", [0.0, 90.0, 180.0])
+def test_camera_orientations(i_obs):
+    check_fp64(*both(0.9, 10, 10, 20.0, 20.0, i_obs, nu=(1.0,)), 1)
+
+
+def test_eight_channels():
+    ref, got = both(0.9375, 9, 9, 24.0, 24.0, 75.0, nu=NUS8)
+    check_fp64(ref, got, 8)
+    _, f32 = both(0.9375, 9, 9, 24.0, 24.0, 75.0, nu=NUS8, precision=G.FP32)
+    I0, I1 = ref[0][:, 8:16], f32[0][:, 8:16]
+    l1 = np.abs(I1 - I0).sum(0) / np.abs(I0).sum(0)
+    assert (l1 <= 1e-4).all(), l1
+
+
+def test_plane_inside_the_shadow():
+    """A 2 M x 2 M plane at the centre of the shadow: every ray is captured
+    and stops at r < 1.01 r_H well before the last interval."""
+    ref, got = both(0.5, 8, 8, 2.0, 2.0, 30.0, n_dump=64)
+    check_fp64(ref, got, 0)
+    r = np.sqrt(np.sum(got[0][:, 1:4] ** 2, axis=1))
+    assert (r < 2.0).all()
+    assert (got[2] < 64 * 1024).all()
