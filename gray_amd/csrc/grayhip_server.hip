@@ -13,6 +13,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "launch_serve.h"
@@ -93,6 +94,10 @@ struct grayhip_server {
 	int64_t next_id;
 	int64_t epochs;
 	unsigned long long steps[3];
+	/* Guards the host bookkeeping (requests, queues, arena, slot lists) so
+	 * submit / next_done / state / result / stats may run on other threads
+	 * while one thread steps; not held while an epoch runs on the GPU. */
+	std::mutex mu;
 	char err[256];
 };
 
@@ -246,7 +251,6 @@ extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q
 		                                "device and no transfer channels");
 	HReq r;
 	memset(&r, 0, sizeof(r));
-	r.id = v->next_id++;
 	r.state = GRAYHIP_REQ_QUEUED;
 	r.tab = -1;
 	r.kind = q->spacetime ? SRV_GRID : SRV_KERR;
@@ -292,6 +296,8 @@ extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q
 		               .view<float>();
 	r.dev.kind = r.kind;
 	r.dev.nd = r.nd;
+	std::lock_guard<std::mutex> lk(v->mu);
+	r.id = v->next_id++;
 	v->reqs[r.id] = r;
 	v->pending.push_back(r.id);
 	*id = r.id;
@@ -407,15 +413,19 @@ extern "C" int grayhip_server_step(grayhip_server *v, int *busy)
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
 	SCK(v, hipSetDevice(v->cfg.device));
-	int rc = admit(v);
 	int n_assign = 0;
-	if (!rc)
-		rc = refill(v, &n_assign);
-	if (rc)
-		return rc;
-	const bool work = !v->list[0].empty() || !v->list[1].empty() || !v->list[2].empty();
-	if (busy)
-		*busy = work || !v->pending.empty();
+	bool work;
+	{
+		std::lock_guard<std::mutex> lk(v->mu);
+		int rc = admit(v);
+		if (!rc)
+			rc = refill(v, &n_assign);
+		if (rc)
+			return rc;
+		work = !v->list[0].empty() || !v->list[1].empty() || !v->list[2].empty();
+		if (busy)
+			*busy = work || !v->pending.empty();
+	}
 	if (!work)
 		return GRAYHIP_OK;
 	const size_t S = (size_t)v->S.S;
@@ -435,6 +445,7 @@ extern "C" int grayhip_server_step(grayhip_server *v, int *busy)
 	SCK(v, launch_srv_epoch(2, v->d_reqs, v->S, v->d_list[2], (int)v->list[2].size(), K, v->L, v->s[2]));
 	for (int k = 0; k < 3; ++k)
 		SCK(v, hipStreamSynchronize(v->s[k]));
+	std::lock_guard<std::mutex> lk(v->mu);
 	++v->epochs;
 	return harvest(v);
 }
@@ -443,6 +454,7 @@ extern "C" int grayhip_server_next_done(grayhip_server *v, int64_t *id)
 {
 	if (!v || !id)
 		return vfail(v, GRAYHIP_EINVAL, "null argument");
+	std::lock_guard<std::mutex> lk(v->mu);
 	if (v->done.empty())
 		return 1;
 	*id = v->done.front();
@@ -454,6 +466,7 @@ extern "C" int grayhip_server_state(grayhip_server *v, int64_t id, int *state)
 {
 	if (!v || !state)
 		return vfail(v, GRAYHIP_EINVAL, "null argument");
+	std::lock_guard<std::mutex> lk(v->mu);
 	auto it = v->reqs.find(id);
 	if (it == v->reqs.end())
 		return vfail(v, GRAYHIP_EINVAL, "unknown request %lld", (long long)id);
@@ -466,6 +479,7 @@ extern "C" int grayhip_server_result(grayhip_server *v, int64_t id, double *data
 {
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
+	std::lock_guard<std::mutex> lk(v->mu);
 	auto it = v->reqs.find(id);
 	if (it == v->reqs.end())
 		return vfail(v, GRAYHIP_EINVAL, "unknown request %lld", (long long)id);
@@ -489,6 +503,7 @@ extern "C" int grayhip_server_stats(grayhip_server *v, int64_t steps[3], int64_t
 {
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
+	std::lock_guard<std::mutex> lk(v->mu);
 	if (steps)
 		for (int k = 0; k < 3; ++k)
 			steps[k] = (int64_t)v->steps[k];

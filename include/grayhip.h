@@ -215,7 +215,12 @@ const char *grayhip_spacetime_error(const grayhip_spacetime *st);
  * in the FP32 form and move to FP64 re-integration when their step
  * collapses; gridded rays run in the FP32 form on resident spacetimes.  A
  * ray's arithmetic does not depend on which slot, epoch or neighbours it
- * gets, so a request's result is the same alone and in any mix. */
+ * gets, so a request's result is the same alone and in any mix.
+ *
+ * Threads: one thread calls grayhip_server_step; submit, next_done, state,
+ * result and stats may be called from other threads at the same time (the
+ * host bookkeeping is locked, and the lock is not held while an epoch runs
+ * on the GPU).  create and destroy are not concurrent with anything. */
 typedef struct grayhip_server grayhip_server;
 
 typedef struct {

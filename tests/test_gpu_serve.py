@@ -109,3 +109,35 @@ def test_admission_waits_for_budget():
         np.testing.assert_array_equal(out[a][0], out[b][0])
         with pytest.raises(G.GrayHipError):
             srv.submit(G.kerr_request(64, 0.5, 50.0, NUS))  # larger than the budget
+
+
+def test_submit_from_another_thread_while_stepping(grid):
+    """One thread steps the pool while another submits the mix over time;
+    every request finishes with the same result as in a single-thread run."""
+    import threading
+    import time
+    reqs = mix(grid)
+    ref, _, _ = run_all(reqs, slots=1500, epoch_steps=700)
+    with G.Server(slots=1500, epoch_steps=700) as srv:
+        ids = []
+
+        def feed():
+            for q in reqs:
+                ids.append(srv.submit(q))
+                time.sleep(0.002)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        out = {}
+        while th.is_alive() or len(out) < len(reqs):
+            srv.step()
+            while True:
+                rid = srv.next_done()
+                if rid is None:
+                    break
+                out[rid] = srv.result(rid)
+        th.join()
+    assert len(out) == len(reqs)
+    for k, rid in enumerate(ids):
+        for a, b in zip(ref[k], out[rid]):
+            assert np.array_equal(a, b), f"request {k} differs"
