@@ -300,23 +300,36 @@ static GridDesc grid_view(const grayhip_ctx *c)
 	                      c->cfg.escape == GRAYHIP_ESC_REF);
 }
 
-/* Numerical spacetime: one launch per interval in either form; the FP32
- * form's diagnostics are recomputed in FP64 at the end. */
+/* Numerical spacetime: the plain FP64 form launches once per interval; the
+ * FP32 form runs its tile kernel (LDS-staged stencils) and its diagnostics
+ * are recomputed in FP64 at the end. */
 static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
 {
 	if (c->rt.n_freq > 0 && !grayhip_spacetime_has_rho(c->st))
 		return fail(c, GRAYHIP_ESTATE, "transfer needs the density of every level in use "
 		                               "(grayhip_spacetime_upload_rho)");
 	const GridDesc g = grid_view(c);
+	/* GRAYHIP_GRID_BRICKS=0 turns the LDS bricks off (a diagnostic: the
+	 * results are the same either way) */
+	const char *env_b = getenv("GRAYHIP_GRID_BRICKS");
+	const bool bricks = !(env_b && env_b[0] == '0');
+	int launches = 0;
 	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
 	HCK(c, hipEventRecord(c->ev[0], c->stream));
-	for (size_t d = 0; d < n_dump; ++d) {
-		if (c->cfg.precision == GRAYHIP_FP64)
+	if (c->cfg.precision == GRAYHIP_FP64) {
+		for (size_t d = 0; d < n_dump; ++d)
 			HCK(c, launch_grid_evolve_f64(g, c->rt, one, c->d_data, c->d_info, c->d_steps,
 			                              c->n_rays, c->stream));
-		else
-			HCK(c, launch_grid_evolve_f32(g, c->rt, one, c->d_data, c->d_steps, c->n_rays,
-			                              &c->d_ctr->pool_steps, c->stream));
+		launches = (int)n_dump;
+	} else {
+		/* all intervals in one launch: one front of rays, no per-interval
+		 * tail and no FP64 round trip of the state between intervals */
+		StepRule all = one;
+		all.n_dump = (int)n_dump;
+		HCK(c, launch_grid_tiles_f32(g, c->rt, all, c->d_data, c->d_steps, c->n_rays,
+		                             (int)c->cfg.w_rays, (int)c->cfg.h_rays, bricks,
+		                             &c->d_ctr->pool_steps, c->stream));
+		launches = 1;
 	}
 	HCK(c, hipEventRecord(c->ev[1], c->stream));
 	if (c->cfg.precision != GRAYHIP_FP64)
@@ -327,7 +340,7 @@ static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *
 	float ms = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
 	c->kern_ns = 1e6 * (double)ms;
-	c->kern_launches = (int)n_dump;
+	c->kern_launches = launches;
 	c->last_fallback = 0;
 	if (ns)
 		*ns = 1e6 * (double)ms;
@@ -425,7 +438,6 @@ static int run_fp64(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *
 	c->kern_ns = 1e6 * (double)ms;
 	c->kern_launches = (int)n_dump;
 	c->last_fallback = 0;
-	memset(&c->last_ctr, 0, sizeof(c->last_ctr));
 	if (ns)
 		*ns = 1e6 * (double)ms;
 	return GRAYHIP_OK;

@@ -51,6 +51,7 @@ __device__ __forceinline__ int sym(int a, int b)
  * the last point (sx, sy, sz = 0 there). */
 template <class T> struct Stencil {
 	uint32_t base, sx, sy, sz;
+	int c[3];        /* lower corner (cell) indices i0, j0, k0 */
 	T f[3];
 	T wt;            /* time weight; < 0: space only */
 
@@ -115,6 +116,9 @@ __device__ __forceinline__ void make_stencil(const GridView<S> &G, T qt, T qx, T
 	st.sx = i[0] + 1 < G.n[0] ? 1u : 0u;
 	st.sy = i[1] + 1 < G.n[1] ? nx : 0u;
 	st.sz = i[2] + 1 < G.n[2] ? nxy : 0u;
+	st.c[0] = i[0];
+	st.c[1] = i[1];
+	st.c[2] = i[2];
 	st.f[0] = f[0];
 	st.f[1] = f[1];
 	st.f[2] = f[2];

@@ -37,6 +37,32 @@ __device__ __forceinline__ void load40(const double *__restrict__ a, float v[40]
 	}
 }
 
+/* Half a point's Christoffels (mu = 0, 1 or mu = 2, 3): a corner is read
+ * in two halves so that only 20 values are live at a time. */
+__device__ __forceinline__ void load20(const float *__restrict__ a, float v[20])
+{
+	const float4 *a4 = reinterpret_cast<const float4 *>(a);
+#pragma unroll
+	for (int i = 0; i < 5; ++i) {
+		float4 t = a4[i];
+		v[4 * i] = t.x;
+		v[4 * i + 1] = t.y;
+		v[4 * i + 2] = t.z;
+		v[4 * i + 3] = t.w;
+	}
+}
+
+__device__ __forceinline__ void load20(const double *__restrict__ a, float v[20])
+{
+	const double2 *a2 = reinterpret_cast<const double2 *>(a);
+#pragma unroll
+	for (int i = 0; i < 10; ++i) {
+		double2 t = a2[i];
+		v[2 * i] = (float)t.x;
+		v[2 * i + 1] = (float)t.y;
+	}
+}
+
 /* sum_k w_k (Gamma^mu_ab(corner k) P_ab) for one level, mu = 0..3 */
 template <class S>
 __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const Stencil<float> &st,
@@ -47,16 +73,19 @@ __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const St
 		out[mu] = 0.f;
 #pragma unroll 1
 	for (int k = 0; k < 8; ++k) {
-		float v[40];
-		load40(lev + (size_t)st.corner(k) * 40, v);
 		const float w = st.weight(k);
 #pragma unroll
-		for (int mu = 0; mu < 4; ++mu) {
-			float t = 0.f;
+		for (int hf = 0; hf < 2; ++hf) {
+			float v[20];
+			load20(lev + (size_t)st.corner(k) * 40 + 20 * hf, v);
 #pragma unroll
-			for (int c = 0; c < 10; ++c)
-				t += v[10 * mu + c] * P[c];
-			out[mu] += w * t;
+			for (int m2 = 0; m2 < 2; ++m2) {
+				float t = 0.f;
+#pragma unroll
+				for (int c = 0; c < 10; ++c)
+					t += v[10 * m2 + c] * P[c];
+				out[2 * hf + m2] += w * t;
+			}
 		}
 	}
 }

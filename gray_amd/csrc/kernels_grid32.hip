@@ -1,6 +1,6 @@
 /*
- * kernels_grid32.hip -- FP32 form of the numerical-spacetime path: one dump
- * interval per launch, FP32 RHS and RK4 stages on a compensated (hi + lo)
+ * kernels_grid32.hip -- FP32 form of the numerical-spacetime path: a
+ * tile kernel, FP32 RHS and RK4 stages on a compensated (hi + lo)
  * state, FP32 or FP64 grid storage; transfer channels (the reference flow)
  * on the state scaled by K as described in rt_thermal.h.
  *
@@ -16,17 +16,266 @@
 
 namespace gray {
 
+/* ------------------------------------------------------------------
+ * Tile kernel of the FP32 gridded form, with LDS-staged interpolation
+ * stencils.
+ *
+ * A wave holds an 8x8 tile of the image (a bundle of neighbouring rays)
+ * and runs every dump interval of its rays in registers; a ray that ends
+ * leaves the wave's active mask, and the wave's decisions below are
+ * ballots over the rays still active.  Every wave keeps a brick of 4x4x4
+ * grid points (3x3x3 cells) of the t1 level's Christoffels in LDS
+ * (64 x 160 B); a lane whose 8 corners lie in the brick reads them from
+ * LDS, the others from global memory.  When more than a quarter of the
+ * wave's rays have left the brick, the wave reloads it around the cell of
+ * its first such ray (one point per lane); if that still leaves half of
+ * them outside (rays spread near the hole), reloading pauses for 8
+ * evaluations.  The brick's origin lives in LDS with it.  LDS holds the
+ * same FP32 values as global memory and the contraction order is the
+ * global path's, so a ray's result does not depend on where its corners
+ * were read (bricks switched off give the same bits).
+ * ------------------------------------------------------------------ */
+
+constexpr int GP_WAVES = 4;                 /* waves per block             */
+constexpr int GP_PTS = 64;                  /* 4x4x4 brick points          */
+
+struct BrickHdr {
+	int o[3];                                /* brick origin (point index)  */
+	int cool;                                /* evaluations without reload  */
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool in_brick(const int c[3], const int o[3])
+{
+	return (unsigned)(c[0] - o[0]) <= 2u && (unsigned)(c[1] - o[1]) <= 2u &&
+	       (unsigned)(c[2] - o[2]) <= 2u;
+}
+
+/* Corner k's point within the brick (upper indices clamped as in the
+ * stencil: a zero stride means the upper corner is the lower one). */
+__device__ __forceinline__ int brick_point(const Stencil<float> &st, const int o[3], int k)
+{
+	const int x = st.c[0] - o[0] + (((k & 1) && st.sx) ? 1 : 0);
+	const int y = st.c[1] - o[1] + (((k & 2) && st.sy) ? 1 : 0);
+	const int z = st.c[2] - o[2] + (((k & 4) && st.sz) ? 1 : 0);
+	return x + 4 * y + 16 * z;
+}
+
+/* lev_contract with the corners read from the LDS brick. */
+__device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, const Stencil<float> &st,
+                                             const int o[3], const float P[10], float out[4])
+{
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+		out[mu] = 0.f;
+#pragma unroll 1
+	for (int k = 0; k < 8; ++k) {
+		const float4 *p = brick + brick_point(st, o, k) * 10;
+		const float w = st.weight(k);
+#pragma unroll
+		for (int hf = 0; hf < 2; ++hf) {
+			float v[20];
+#pragma unroll
+			for (int i = 0; i < 5; ++i) {
+				const float4 q = p[5 * hf + i];
+				v[4 * i] = q.x;
+				v[4 * i + 1] = q.y;
+				v[4 * i + 2] = q.z;
+				v[4 * i + 3] = q.w;
+			}
+#pragma unroll
+			for (int m2 = 0; m2 < 2; ++m2) {
+				float tt = 0.f;
+#pragma unroll
+				for (int c = 0; c < 10; ++c)
+					tt += v[10 * m2 + c] * P[c];
+				out[2 * hf + m2] += w * tt;
+			}
+		}
+	}
+}
+
+/* One point's 40 Christoffels into the brick (as load40 reads them). */
+template <class S>
+__device__ __forceinline__ void brick_store(float4 *brick, int p, const S *src)
+{
+	float v[40];
+	load40(src, v);
+#pragma unroll
+	for (int i = 0; i < 10; ++i)
+		brick[p * 10 + i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+}
+
+/* Gridded RHS with bricks: grid_rhs_all_f32 with the t1 level's corners
+ * from the wave's brick where possible. */
 template <int NF, class S>
-__global__ __launch_bounds__(256) void k_grid_evolve_f32(GridView<S> G, RtThermal rt64, StepRule sr,
-                                                         double *__restrict__ data,
-                                                         int64_t *__restrict__ steps,
-                                                         size_t n_rays,
-                                                         unsigned long long *__restrict__ nsteps)
+__device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThermalF &rt,
+                                              float4 *brick, BrickHdr *hdr, bool use_lds,
+                                              const float *s, float *ds)
+{
+	Stencil<float> st;
+	make_stencil<float, S>(G, s[0], s[1], s[2], s[3], st);
+	const float u0 = s[4], u1 = s[5], u2 = s[6], u3 = s[7];
+	const float P[10] = {u0 * u0, 2.f * u0 * u1, 2.f * u0 * u2, 2.f * u0 * u3, u1 * u1,
+	                     2.f * u1 * u2, 2.f * u1 * u3, u2 * u2, 2.f * u2 * u3, u3 * u3};
+	float uGu[4];
+	bool inside = false;
+	int o[3] = {0, 0, 0};
+	if (use_lds) {
+		/* the header is wave state: re-read it after the previous
+		 * evaluation's writes and take the first active lane's copy, so
+		 * every lane makes the same decisions */
+		wave_sync();
+		o[0] = __builtin_amdgcn_readfirstlane(hdr->o[0]);
+		o[1] = __builtin_amdgcn_readfirstlane(hdr->o[1]);
+		o[2] = __builtin_amdgcn_readfirstlane(hdr->o[2]);
+		const int cool = __builtin_amdgcn_readfirstlane(hdr->cool);
+		inside = in_brick(st.c, o);
+		const unsigned long long act = __ballot(1);
+		const unsigned long long out = __ballot(!inside);
+		const unsigned lane = __lane_id();
+		const int leader = __ffsll((long long)act) - 1;
+		if (cool > 0) {
+			if ((int)lane == leader)
+				hdr->cool = cool - 1;
+			wave_sync();
+		} else if (4 * __popcll(out) > __popcll(act)) {
+			/* reload around the first outside ray's cell */
+			const int first = __ffsll((long long)out) - 1;
+			for (int a = 0; a < 3; ++a) {
+				const int c = __builtin_amdgcn_readfirstlane(__shfl(st.c[a], first));
+				const int hi_o = G.n[a] - 4;
+				o[a] = c - 1 < 0 ? 0 : (c - 1 > hi_o ? hi_o : c - 1);
+			}
+			wave_sync();
+			const int n_act = __popcll(act);
+			const int r = __popcll(act & ((1ull << lane) - 1ull));
+			const uint32_t nx = (uint32_t)G.n[0], nxy = nx * (uint32_t)G.n[1];
+			for (int p = r; p < GP_PTS; p += n_act) {
+				const uint32_t pt = (uint32_t)(o[2] + (p >> 4)) * nxy +
+				                    (uint32_t)(o[1] + ((p >> 2) & 3)) * nx +
+				                    (uint32_t)(o[0] + (p & 3));
+				brick_store<S>(brick, p, G.gam[0] + (size_t)pt * 40);
+			}
+			inside = in_brick(st.c, o);
+			const int still = __popcll(__ballot(!inside));
+			if ((int)lane == leader) {
+				hdr->o[0] = o[0];
+				hdr->o[1] = o[1];
+				hdr->o[2] = o[2];
+				hdr->cool = 2 * still > n_act ? 8 : 0;
+			}
+			wave_sync();
+		}
+	}
+	if (inside)
+		lds_contract(brick, st, o, P, uGu);
+	else
+		lev_contract<S>(G.gam[0], st, P, uGu);
+	if (st.wt > 0.f) {
+		float y2[4];
+		lev_contract<S>(G.gam[1], st, P, y2);
+#pragma unroll
+		for (int mu = 0; mu < 4; ++mu)
+			uGu[mu] = uGu[mu] + st.wt * (y2[mu] - uGu[mu]);
+	}
+	const float u[4] = {u0, u1, u2, u3};
+	ds[0] = u0; ds[1] = u1; ds[2] = u2; ds[3] = u3;
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+		ds[4 + mu] = -uGu[mu] + uGu[0] * u[mu];
+	if constexpr (NF > 0) {
+		float kt, ne;
+		lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
+		if (st.wt > 0.f) {
+			float kt2, ne2;
+			lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
+			kt = kt + st.wt * (kt2 - kt);
+			ne = ne + st.wt * (ne2 - ne);
+		}
+		thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
+	}
+}
+
+/* grid_rk4_f32c over grid_rhs_brick. */
+template <int NF, class S>
+__device__ __forceinline__ void grid_rk4_brick(const GridView<S> &G, const RtThermalF &rt,
+                                              float4 *brick, BrickHdr *hdr, bool use_lds,
+                                              float *hi, float *lo, float dt)
 {
 	constexpr int ND = 8 + 2 * NF;
-	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= n_rays)
+	float k[ND], acc[ND], x[ND];
+	const float hdt = 0.5f * dt;
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, hi, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] = k[i];
+		x[i] = hi[i] + hdt * k[i];
+	}
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] += 2.0f * k[i];
+		x[i] = hi[i] + hdt * k[i];
+	}
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		acc[i] += 2.0f * k[i];
+		x[i] = hi[i] + dt * k[i];
+	}
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+	const float dt6 = dt * (1.0f / 6.0f);
+#pragma unroll
+	for (int i = 0; i < ND; ++i) {
+		float y = (acc[i] + k[i]) * dt6 + lo[i];
+		float tt = hi[i] + y;
+		lo[i] = y - (tt - hi[i]);
+		hi[i] = tt;
+	}
+}
+
+/* One launch = n_dump intervals of every ray; wave w of block b holds tile
+ * 4b + w (8x8 rays).  All resident waves start together and advance at
+ * similar rates, so the rays in flight sweep the grid as one front and the
+ * cells they read stay in L2.  Blocks reach the 8 XCDs round robin, which
+ * gives every XCD an even share of the costly tiles near the hole. */
+template <int NF, class S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF == 0 ? 3 : 1)))
+void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__restrict__ data,
+                      int64_t *__restrict__ steps, size_t n_rays, int W, int H, int bricks,
+                      unsigned long long *__restrict__ n_steps)
+{
+	constexpr int ND = 8 + 2 * NF;
+	__shared__ float4 s_brick[GP_WAVES][GP_PTS * 10];
+	__shared__ BrickHdr s_hdr[GP_WAVES];
+	const unsigned lane = __lane_id();
+	const int wv = (int)(threadIdx.x >> 6);
+	float4 *brick = s_brick[wv];
+	BrickHdr *hdr = &s_hdr[wv];
+	if (lane == 0) {
+		/* an origin no cell matches: the first evaluation loads */
+		hdr->o[0] = hdr->o[1] = hdr->o[2] = -1000000;
+		hdr->cool = 0;
+	}
+	wave_sync();
+
+	const unsigned tw = (unsigned)(W + 7) / 8u, th = (unsigned)(H + 7) / 8u;
+	const unsigned tile = blockIdx.x * GP_WAVES + (unsigned)wv;
+	if (tile >= tw * th)
 		return;
+	const unsigned x = (tile % tw) * 8u + (lane & 7u), y = (tile / tw) * 8u + (lane >> 3);
+	if (x >= (unsigned)W || y >= (unsigned)H)
+		return;
+	const size_t g = (size_t)y * (size_t)W + x;
+
+	const bool use_lds = bricks && G.n[0] >= 4 && G.n[1] >= 4 && G.n[2] >= 4;
 	const RtThermalF rt = rt_thermal_f32(rt64);
 	const double inv_scale = NF > 0 ? 1.0 / rt64.scale : 1.0;
 	float hi[ND], lo[ND];
@@ -34,24 +283,33 @@ __global__ __launch_bounds__(256) void k_grid_evolve_f32(GridView<S> G, RtTherma
 	for (int i = 0; i < ND; ++i) {
 		double v = data[(size_t)i * n_rays + g];
 		if (i >= 8)
-			v *= inv_scale;               /* I' = I / K, tau' = tau / K */
+			v *= inv_scale;
 		hi[i] = (float)v;
 		lo[i] = (float)(v - (double)hi[i]);
 	}
+	int64_t st = steps[g];
+	const int64_t st0 = st;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
-	int64_t nst = 0;
-	int dh;
-	for (int h = 0; h < sr.n; h += dh) {
-		float gdt = grid_getdt<float, S>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
-		double xd = (double)gdt / sr.dt * (double)sr.n;
-		dh = (xd == xd) ? (int)xd : 0;
-		if (!dh)
+	for (int d = 0; d < sr.n_dump; ++d) {
+		int dh;
+		bool retired = false;
+		for (int h = 0; h < sr.n; h += dh) {
+			float gdt = grid_getdt<float, S>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
+			double xd = (double)gdt / sr.dt * (double)sr.n;
+			dh = (xd == xd) ? (int)xd : 0;
+			if (!dh) {
+				retired = true;
+				break;
+			}
+			if (dh > sr.n - h)
+				dh = sr.n - h;
+			grid_rk4_brick<NF, S>(G, rt, brick, hdr, use_lds, hi, lo,
+			                     (float)((double)dh * dt_over_n));
+			++st;
+		}
+		if (retired)
 			break;
-		if (dh > sr.n - h)
-			dh = sr.n - h;
-		grid_rk4_f32c<NF, S>(G, rt, hi, lo, (float)((double)dh * dt_over_n));
-		++nst;
 	}
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
@@ -60,35 +318,44 @@ __global__ __launch_bounds__(256) void k_grid_evolve_f32(GridView<S> G, RtTherma
 			v *= rt64.scale;
 		data[(size_t)i * n_rays + g] = v;
 	}
-	steps[g] += nst;
-	if (nsteps)
-		atomicAdd(nsteps, (unsigned long long)nst);
+	steps[g] = st;
+	atomicAdd(n_steps, (unsigned long long)(st - st0));
 }
 
-static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-template <int NF>
-static void grid_evolve_f32_nf(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
-                               double *data, int64_t *steps, size_t n_rays,
-                               unsigned long long *nsteps, hipStream_t st)
+template <int NF, class S>
+static void grid_tiles_launch(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                              double *data, int64_t *steps, size_t n_rays, int W, int H,
+                              int bricks, unsigned long long *n_steps, unsigned n_blocks,
+                              hipStream_t st)
 {
-	if (gd.dtype == 4)
-		hipLaunchKernelGGL((k_grid_evolve_f32<NF, float>), dim3(blocks_for(n_rays, 256)), dim3(256),
-		                   0, st, gd.view<float>(), rt, sr, data, steps, n_rays, nsteps);
-	else
-		hipLaunchKernelGGL((k_grid_evolve_f32<NF, double>), dim3(blocks_for(n_rays, 256)), dim3(256),
-		                   0, st, gd.view<double>(), rt, sr, data, steps, n_rays, nsteps);
+	hipLaunchKernelGGL((k_grid_tiles_f32<NF, S>), dim3(n_blocks), dim3(256), 0, st,
+	                   gd.view<S>(), rt, sr, data, steps, n_rays, W, H, bricks, n_steps);
 }
 
-hipError_t launch_grid_evolve_f32(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
-                                  double *data, int64_t *steps, size_t n_rays,
-                                  unsigned long long *nsteps, hipStream_t st)
+hipError_t launch_grid_tiles_f32(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                 double *data, int64_t *steps, size_t n_rays, int W, int H,
+                                 bool bricks, unsigned long long *n_steps, hipStream_t st)
 {
 	if (n_rays == 0)
 		return hipSuccess;
-#define L(NF) grid_evolve_f32_nf<NF>(gd, rt, sr, data, steps, n_rays, nsteps, st)
-	GRAY_NF_SWITCH(rt.n_freq, L)
+	if (W <= 0 || H <= 0 || (size_t)W * (size_t)H != n_rays)
+		return hipErrorInvalidValue;
+	const size_t tiles = (((size_t)W + 7) / 8) * (((size_t)H + 7) / 8);
+	const size_t nb = (tiles + GP_WAVES - 1) / GP_WAVES;
+	if (nb > (size_t)INT_MAX)
+		return hipErrorInvalidValue;
+	const int b = bricks ? 1 : 0;
+	if (gd.dtype == 4) {
+#define L(NF) grid_tiles_launch<NF, float>(gd, rt, sr, data, steps, n_rays, W, H, b, n_steps, \
+	                                   (unsigned)nb, st)
+		GRAY_NF_SWITCH(rt.n_freq, L)
 #undef L
+	} else {
+#define L(NF) grid_tiles_launch<NF, double>(gd, rt, sr, data, steps, n_rays, W, H, b, n_steps, \
+	                                    (unsigned)nb, st)
+		GRAY_NF_SWITCH(rt.n_freq, L)
+#undef L
+	}
 	return hipGetLastError();
 }
 

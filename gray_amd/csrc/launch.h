@@ -59,8 +59,12 @@ hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *
                                 hipStream_t st);
 hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,
                                   double *out, int n, hipStream_t st);
-hipError_t launch_grid_evolve_f32(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
-                                  double *data, int64_t *steps, size_t n_rays,
-                                  unsigned long long *nsteps, hipStream_t st);
+/* FP32 gridded form, n_dump intervals of every ray in one launch (8x8-ray
+ * tiles, one per wave).  bricks = false makes every lane read its stencils
+ * from global memory (same values, same order: the results do not change).
+ * Steps taken are added to *n_steps. */
+hipError_t launch_grid_tiles_f32(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                 double *data, int64_t *steps, size_t n_rays, int W, int H,
+                                 bool bricks, unsigned long long *n_steps, hipStream_t st);
 
 } // namespace gray

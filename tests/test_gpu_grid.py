@@ -204,3 +204,49 @@ def test_launcher_dyst_sliding_window_matches_oracle(tmp_path):
         assert np.array_equal(states.reshape(-1, 8), data), i
         assert np.array_equal(diag.ravel(), info), i
     assert seen == [3, 2, 0, 0]
+
+
+def test_fp32_render_matches_interval_launches():
+    """The FP32 form's one launch over 3 intervals against 3 launches of one
+    interval each.  Between launches the compensated state is stored as
+    hi + lo in FP64, which rounds a lo more than 2^29 below hi, so the two
+    agree to FP32-form accuracy, not bit for bit."""
+    n, times = 33, (0.0, 1.0)
+    xs, lv = make_grid(n, 4, times=times)
+    st = device_grid(n, 4, lv, times)
+    out = []
+    for per_launch in (False, True):
+        with G.Render(20, 12, precision=G.FP32, spacetime=st, t_init=1.0) as r:
+            r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+            if per_launch:
+                for _ in range(3):
+                    r.evolve(-32.0, 1024)
+            else:
+                r.render(-32.0, 1024, 3)
+            out.append((r.read_f64()[0], r.steps()))
+    st.close()
+    (d0, s0), (d1, s1) = out
+    same = s0 == s1
+    assert same.mean() > 0.95
+    dx = np.abs(d1[same, :4] - d0[same, :4]).max(axis=1)
+    assert dx.max() < 1e-4, dx.max()
+
+
+@pytest.mark.parametrize("dtype", [4, 8])
+def test_fp32_bricks_do_not_change_results(dtype, monkeypatch):
+    """The FP32 tile kernel's LDS bricks hold the grid's own values and are
+    contracted in the global path's order: the render with bricks equals the
+    render with every corner read from global memory, bit for bit."""
+    n, times = 33, (0.0, 1.0)
+    xs, lv = make_grid(n, dtype, times=times)
+    st = device_grid(n, dtype, lv, times)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GRAYHIP_GRID_BRICKS", flag)
+        with G.Render(40, 40, precision=G.FP32, spacetime=st, t_init=1.0) as r:
+            r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+            r.render(-32.0, 1024, 3)
+            out.append((r.read_f64()[0], r.steps()))
+    st.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])

@@ -135,6 +135,22 @@ def test_thermal_fp32_form_vs_fp64():
         assert (l1 <= 1e-4).all(), l1
 
 
+def test_thermal_fp32_bricks_do_not_change_results(monkeypatch):
+    """Transfer rides along the FP32 tile kernel's LDS-staged geodesics: the
+    render with bricks equals the one with every corner read from global
+    memory, bit for bit, channels included."""
+    n, times = 33, (0.0, 1.0)
+    st = device(n, boosted(n, times, 4), times, 4)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GRAYHIP_GRID_BRICKS", flag)
+        out.append(gpu_render(st, G.FP32, 32, t_init=1.0, n_dump=3))
+    st.close()
+    (d0, i0, s0), (d1, i1, s1) = out
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1, equal_nan=True)
+
+
 def test_launcher_dyst_job_with_transfer(tmp_path):
     exe = os.path.join(os.path.dirname(J.LIB_PATH), "gray-mkgrid")
     f = str(tmp_path / "bks.h5")
