@@ -169,6 +169,7 @@ GridDesc spacetime_view(const grayhip_spacetime *s, double t_off, int raw_icond,
 		g.lo[a] = s->desc.lo[a];
 		g.hi[a] = s->desc.hi[a];
 		g.dx[a] = (g.hi[a] - g.lo[a]) / (double)(g.n[a] - 1);
+		g.idx[a] = (double)(g.n[a] - 1) / (g.hi[a] - g.lo[a]);
 		g.fb[a] = cbrt(asinh(g.lo[a]));
 		g.fa[a] = (cbrt(asinh(g.hi[a])) - g.fb[a]) / (double)(g.n[a] - 1);
 	}

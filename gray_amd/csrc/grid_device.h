@@ -30,6 +30,7 @@ struct GridView {
 	int    map;           /* GRAYHIP_MAP_*                             */
 	double lo[3], hi[3];
 	double dx[3];         /* (hi - lo) / (n - 1), uniform map           */
+	double idx[3];        /* (n - 1) / (hi - lo), uniform map           */
 	double fa[3], fb[3];  /* fisheye A, B per axis                      */
 	double t1, t2;        /* level times; equal: space only             */
 	double t_off;         /* grid time at q^t = 0                       */
@@ -85,6 +86,19 @@ __device__ __forceinline__ void locate(const GridView<S> &G, int ax, T x, int &i
 		T c0 = A * T(i) + B, c1 = A * T(i + 1) + B;
 		x0 = sinh((c0 * c0) * c0);
 		x1 = sinh((c1 * c1) * c1);
+	} else if constexpr (sizeof(T) == 4) {
+		/* FP32 form: on a uniform axis i + (x - x0) / (x1 - x0) is
+		 * (x - lo) / dx, formed with the host's 1/dx (one multiply; the
+		 * plain FP64 form keeps the reference's two divisions) */
+		T u = (x - T(G.lo[ax])) * T(G.idx[ax]);
+		if (!(u > T(0)))
+			u = T(0);
+		if (u > T(n - 1))
+			u = T(n - 1);
+		T fl = ex_floor(u);
+		i0 = (int)fl;
+		f = u - fl;
+		return;
 	} else {
 		const T lo = T(G.lo[ax]), dx = T(G.dx[ax]);
 		xi = ex_floor((x - lo) / dx);
@@ -167,7 +181,7 @@ struct GridDesc {
 	const void *rho[2];           /* densities of the slots (may be null)  */
 	int n[3];
 	int map;
-	double lo[3], hi[3], dx[3], fa[3], fb[3];
+	double lo[3], hi[3], dx[3], idx[3], fa[3], fb[3];
 	double t1, t2, t_off;
 	int raw_icond, ref_rule;
 
@@ -184,6 +198,7 @@ struct GridDesc {
 			v.lo[a] = lo[a];
 			v.hi[a] = hi[a];
 			v.dx[a] = dx[a];
+			v.idx[a] = idx[a];
 			v.fa[a] = fa[a];
 			v.fb[a] = fb[a];
 		}

@@ -57,27 +57,25 @@ __device__ __forceinline__ bool in_brick(const int c[3], const int o[3])
 	       (unsigned)(c[2] - o[2]) <= 2u;
 }
 
-/* Corner k's point within the brick (upper indices clamped as in the
- * stencil: a zero stride means the upper corner is the lower one). */
-__device__ __forceinline__ int brick_point(const Stencil<float> &st, const int o[3], int k)
-{
-	const int x = st.c[0] - o[0] + (((k & 1) && st.sx) ? 1 : 0);
-	const int y = st.c[1] - o[1] + (((k & 2) && st.sy) ? 1 : 0);
-	const int z = st.c[2] - o[2] + (((k & 4) && st.sz) ? 1 : 0);
-	return x + 4 * y + 16 * z;
-}
-
-/* lev_contract with the corners read from the LDS brick. */
+/* lev_contract with the corners read from the LDS brick: the 8 corners
+ * unrolled, their brick offsets and weights formed once (upper indices
+ * clamped as in the stencil: a zero stride means the upper corner is the
+ * lower one).  Weights and sums are the stencil's, in its order. */
 __device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, const Stencil<float> &st,
                                              const int o[3], const float P[10], float out[4])
 {
+	const float4 *p0 = brick + ((st.c[0] - o[0]) + 4 * (st.c[1] - o[1]) + 16 * (st.c[2] - o[2])) * 10;
+	const int ox = st.sx ? 10 : 0, oy = st.sy ? 40 : 0, oz = st.sz ? 160 : 0;
+	const float gx[2] = {1.f - st.f[0], st.f[0]};
+	const float gy[2] = {1.f - st.f[1], st.f[1]};
+	const float gz[2] = {1.f - st.f[2], st.f[2]};
 #pragma unroll
 	for (int mu = 0; mu < 4; ++mu)
 		out[mu] = 0.f;
-#pragma unroll 1
+#pragma unroll
 	for (int k = 0; k < 8; ++k) {
-		const float4 *p = brick + brick_point(st, o, k) * 10;
-		const float w = st.weight(k);
+		const float4 *p = p0 + ((k & 1) ? ox : 0) + ((k & 2) ? oy : 0) + ((k & 4) ? oz : 0);
+		const float w = (gx[k & 1] * gy[(k >> 1) & 1]) * gz[k >> 2];
 #pragma unroll
 		for (int hf = 0; hf < 2; ++hf) {
 			float v[20];
