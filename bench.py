@@ -57,6 +57,14 @@ FLOP_PER_STEP = F_KS + f_rt(len(NUS))
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
+# HBM bytes per launch of the dominant kernels on these workloads, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the same
+# counters read for a 256 MiB device copy (read x1.9997, write x1.0000):
+# profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
+# profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float>).
+TRAFFIC_POOL_PASS = (83.67 + 81.84) * 2**20
+TRAFFIC_GRID_TILES = (628.58 + 37.49) * 2**20
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
 # Algorithmic bytes per gridded ray-step: 4 RHS x 40 Christoffels x 8
 # corners x 2 time levels x bytes per stored real.
 GRID_B = {4: 4 * 40 * 8 * 2 * 4, 8: 4 * 40 * 8 * 2 * 8}
@@ -117,7 +125,8 @@ def headline(G, args, out):
         "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
         "frac_of_unpacked_fp32": achieved / PEAK_FP64_TFLOPS,
-        "traffic": None,
+        "traffic": TRAFFIC_POOL_PASS, "traffic_unit": "B per pass",
+        "traffic_source": TRAFFIC_SOURCE,
         "flop_per_ray_step": FLOP_PER_STEP,
         "kernel_ms": 1e3 * k_s,
     }
@@ -176,7 +185,9 @@ def grid_section(G, out):
             "unit": "ray-steps/s", "ray_steps": steps, "render_ms": 1e3 * wall,
             "roofline": {"bound": "hbm", "kernel": "k_grid_tiles_f32<0, float>" if dtype == 4 else "k_grid_evolve_f64<0, double>",
                          "achieved": gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                         "frac": gbps / PEAK_HBM_GBPS, "traffic": None,
+                         "frac": gbps / PEAK_HBM_GBPS,
+                         "traffic": TRAFFIC_GRID_TILES if dtype == 4 else None,
+                         "traffic_source": TRAFFIC_SOURCE if dtype == 4 else None,
                          "bytes_per_ray_step": GRID_B[dtype]},
         }
     (d32, s32), (d64, s64) = finals[4], finals[8]
