@@ -183,7 +183,7 @@ def grid_section(G, out):
         res[f"{'fp32' if dtype == 4 else 'fp64'}_grid"] = {
             "form": "fp32" if prec == G.FP32 else "fp64", "value": steps / wall,
             "unit": "ray-steps/s", "ray_steps": steps, "render_ms": 1e3 * wall,
-            "roofline": {"bound": "hbm", "kernel": "k_grid_tiles_f32<0, float>" if dtype == 4 else "k_grid_evolve_f64<0, double>",
+            "roofline": {"bound": "hbm", "kernel": "k_grid_tiles_f32<0, float>" if dtype == 4 else "k_grid_tiles_f64<0, double>",
                          "achieved": gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / PEAK_HBM_GBPS,
                          "traffic": TRAFFIC_GRID_TILES if dtype == 4 else None,

@@ -300,9 +300,9 @@ static GridDesc grid_view(const grayhip_ctx *c)
 	                      c->cfg.escape == GRAYHIP_ESC_REF);
 }
 
-/* Numerical spacetime: the plain FP64 form launches once per interval; the
- * FP32 form runs its tile kernel (LDS-staged stencils) and its diagnostics
- * are recomputed in FP64 at the end. */
+/* Numerical spacetime: both forms run every interval of the call in one
+ * launch of their tile kernel (LDS-staged stencils); the FP32 form's
+ * diagnostics are recomputed in FP64 at the end. */
 static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
 {
 	if (c->rt.n_freq > 0 && !grayhip_spacetime_has_rho(c->st))
@@ -313,23 +313,21 @@ static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *
 	 * results are the same either way) */
 	const char *env_b = getenv("GRAYHIP_GRID_BRICKS");
 	const bool bricks = !(env_b && env_b[0] == '0');
-	int launches = 0;
 	HCK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(Counters), c->stream));
 	HCK(c, hipEventRecord(c->ev[0], c->stream));
+	/* all intervals in one launch: one front of rays, no per-interval tail
+	 * (the FP64 state stays in registers: the same bits as one launch per
+	 * interval) */
+	StepRule all = one;
+	all.n_dump = (int)n_dump;
 	if (c->cfg.precision == GRAYHIP_FP64) {
-		for (size_t d = 0; d < n_dump; ++d)
-			HCK(c, launch_grid_evolve_f64(g, c->rt, one, c->d_data, c->d_info, c->d_steps,
-			                              c->n_rays, c->stream));
-		launches = (int)n_dump;
+		HCK(c, launch_grid_tiles_f64(g, c->rt, all, c->d_data, c->d_info, c->d_steps, c->n_rays,
+		                             (int)c->cfg.w_rays, (int)c->cfg.h_rays, bricks, c->stream));
 	} else {
-		/* all intervals in one launch: one front of rays, no per-interval
-		 * tail and no FP64 round trip of the state between intervals */
-		StepRule all = one;
-		all.n_dump = (int)n_dump;
+		/* no FP64 round trip of the compensated state between intervals */
 		HCK(c, launch_grid_tiles_f32(g, c->rt, all, c->d_data, c->d_steps, c->n_rays,
 		                             (int)c->cfg.w_rays, (int)c->cfg.h_rays, bricks,
 		                             &c->d_ctr->pool_steps, c->stream));
-		launches = 1;
 	}
 	HCK(c, hipEventRecord(c->ev[1], c->stream));
 	if (c->cfg.precision != GRAYHIP_FP64)
@@ -340,7 +338,7 @@ static int run_grid(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *
 	float ms = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
 	c->kern_ns = 1e6 * (double)ms;
-	c->kern_launches = launches;
+	c->kern_launches = 1;
 	c->last_fallback = 0;
 	if (ns)
 		*ns = 1e6 * (double)ms;

@@ -144,6 +144,23 @@ __device__ __forceinline__ void make_stencil(const GridView<S> &G, T qt, T qx, T
 	}
 }
 
+/* LDS bricks: a wave's copy of 4x4x4 grid points (3x3x3 cells) from
+ * origin o; a cell c lies in it when 0 <= c - o <= 2 on every axis. */
+__device__ __forceinline__ bool in_brick(const int c[3], const int o[3])
+{
+	return (unsigned)(c[0] - o[0]) <= 2u && (unsigned)(c[1] - o[1]) <= 2u &&
+	       (unsigned)(c[2] - o[2]) <= 2u;
+}
+
+/* Orders a wave's LDS writes before its later reads (and reads before
+ * later writes): wave-scope fences around a scheduling barrier. */
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* Coordinate speed |dx/dt|^2 above which a ray counts as captured: the
  * horizon proxy of the gridded path.  Outside a horizon light moves at
  * |dx/dt| of order 1; deep inside, near the singularity, the interpolated

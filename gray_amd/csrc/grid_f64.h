@@ -1,7 +1,8 @@
 /*
  * grid_f64.h -- FP64 interpolation of a resident spacetime: per-field
- * trilinear sums in corner order, time blend, the dyst RHS and g(u,u), in
- * the evaluation order of the FP64 grid oracle.
+ * trilinear sums in corner order, time blend and g(u,u), in the evaluation
+ * order of the FP64 grid oracle (the dyst RHS over them is in
+ * kernels_grid64.hip).
  */
 #pragma once
 
@@ -50,59 +51,6 @@ __device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
 #pragma unroll
 		for (int c = 0; c < NC; ++c)
 			out[c] = out[c] + st.wt * (y2[c] - out[c]);
-	}
-}
-
-template <class S>
-__device__ __forceinline__ void grid_rhs_at_f64(const GridView<S> &G, const Stencil<double> &st,
-                                                const double *s, double *du)
-{
-	const double u[4] = {s[4], s[5], s[6], s[7]};
-	double uGu[4];
-#pragma unroll 1
-	for (int mu = 0; mu < 4; ++mu) {
-		double g[10], v[4];
-		sample_f64<10, 40, S>(G.gam, 10 * mu, st, g);
-#pragma unroll
-		for (int a = 0; a < 4; ++a)
-			v[a] = ((g[sym(a, 0)] * u[0] + g[sym(a, 1)] * u[1]) + g[sym(a, 2)] * u[2]) +
-			       g[sym(a, 3)] * u[3];
-		const double r = ((u[0] * v[0] + u[1] * v[1]) + u[2] * v[2]) + u[3] * v[3];
-		uGu[0] = mu == 0 ? r : uGu[0];
-		uGu[1] = mu == 1 ? r : uGu[1];
-		uGu[2] = mu == 2 ? r : uGu[2];
-		uGu[3] = mu == 3 ? r : uGu[3];
-	}
-#pragma unroll
-	for (int mu = 0; mu < 4; ++mu)
-		du[mu] = -uGu[mu] + uGu[0] * u[mu];
-}
-
-template <class S>
-__device__ __forceinline__ void grid_rhs_f64(const GridView<S> &G, const double *s, double *du)
-{
-	Stencil<double> st;
-	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
-	grid_rhs_at_f64<S>(G, st, s, du);
-}
-
-/* The whole state derivative: geodesic part, then (NF > 0) the transfer
- * along the reference flow: n_e = interpolated density, shift = -k_t with
- * k_t = g_{t nu} u^nu (fluid at rest; sim-org/flow.cl:50-54). */
-template <int NF, class S>
-__device__ __forceinline__ void grid_rhs_all_f64(const GridView<S> &G, const RtThermal &rt,
-                                                 const double *s, double *ds)
-{
-	Stencil<double> st;
-	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
-	ds[0] = s[4]; ds[1] = s[5]; ds[2] = s[6]; ds[3] = s[7];
-	grid_rhs_at_f64<S>(G, st, s, ds + 4);
-	if constexpr (NF > 0) {
-		double m[4], ne;
-		sample_f64<4, 10, S>(G.met, 0, st, m);
-		sample_f64<1, 1, S>(G.rho, 0, st, &ne);
-		const double kt = ((m[0] * s[4] + m[1] * s[5]) + m[2] * s[6]) + m[3] * s[7];
-		thermal_f64<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
 	}
 }
 

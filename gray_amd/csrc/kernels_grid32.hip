@@ -44,19 +44,6 @@ struct BrickHdr {
 	int cool;                                /* evaluations without reload  */
 };
 
-__device__ __forceinline__ void wave_sync()
-{
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ bool in_brick(const int c[3], const int o[3])
-{
-	return (unsigned)(c[0] - o[0]) <= 2u && (unsigned)(c[1] - o[1]) <= 2u &&
-	       (unsigned)(c[2] - o[2]) <= 2u;
-}
-
 /* lev_contract with the corners read from the LDS brick: the 8 corners
  * unrolled, their brick offsets and weights formed once (upper indices
  * clamped as in the stencil: a zero stride means the upper corner is the

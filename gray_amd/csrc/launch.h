@@ -52,9 +52,12 @@ hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data,
 
 hipError_t launch_grid_icond(const GridDesc &gd, const RayParams &p, double *data, double *info,
                              int64_t *steps, size_t n_rays, hipStream_t st);
-hipError_t launch_grid_evolve_f64(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
-                                  double *data, double *info, int64_t *steps, size_t n_rays,
-                                  hipStream_t st);
+/* Plain FP64 gridded form, n_dump intervals of every ray in one launch
+ * (8x8-ray tiles, one per wave, LDS bricks unless bricks = false; the
+ * results are the oracle's either way). */
+hipError_t launch_grid_tiles_f64(const GridDesc &gd, const RtThermal &rt, const StepRule &sr,
+                                 double *data, double *info, int64_t *steps, size_t n_rays, int W,
+                                 int H, bool bricks, hipStream_t st);
 hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *info, size_t n_rays,
                                 hipStream_t st);
 hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,

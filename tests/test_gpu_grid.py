@@ -232,21 +232,26 @@ def test_fp32_render_matches_interval_launches():
     assert dx.max() < 1e-4, dx.max()
 
 
+@pytest.mark.parametrize("precision", [G.FP32, G.FP64])
 @pytest.mark.parametrize("dtype", [4, 8])
-def test_fp32_bricks_do_not_change_results(dtype, monkeypatch):
-    """The FP32 tile kernel's LDS bricks hold the grid's own values and are
-    contracted in the global path's order: the render with bricks equals the
-    render with every corner read from global memory, bit for bit."""
+def test_bricks_do_not_change_results(dtype, precision, monkeypatch):
+    """The tile kernels' LDS bricks hold the grid's own values and are summed
+    in the global path's order: the render with bricks equals the render
+    with every corner read from global memory, bit for bit (40x36 rays: 30
+    tiles of 8x8, the last row of tiles half outside the image)."""
     n, times = 33, (0.0, 1.0)
     xs, lv = make_grid(n, dtype, times=times)
     st = device_grid(n, dtype, lv, times)
     out = []
     for flag in ("1", "0"):
         monkeypatch.setenv("GRAYHIP_GRID_BRICKS", flag)
-        with G.Render(40, 40, precision=G.FP32, spacetime=st, t_init=1.0) as r:
+        with G.Render(40, 36, precision=precision, spacetime=st, t_init=1.0) as r:
             r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
             r.render(-32.0, 1024, 3)
-            out.append((r.read_f64()[0], r.steps()))
+            out.append((r.read_f64(), r.steps()))
     st.close()
-    assert np.array_equal(out[0][1], out[1][1])
-    assert np.array_equal(out[0][0], out[1][0])
+    (d0, i0), s0 = out[0]
+    (d1, i1), s1 = out[1]
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1)
+    assert np.array_equal(i0, i1)
