@@ -211,10 +211,12 @@ int gray_serve_run(const gray_serve_opts *o, gray_serve_report *rep, double *lat
 			}
 			grayhip_server_result(srv, id, NULL, NULL, NULL);
 			++n_done;
-			if (o->verbose)
+			if (o->verbose) {
 				printf("request %d (%dx%d %s) done: %.1f ms\n", k, it[k].side, it[k].side,
 				       it[k].grid >= 0 ? "grid" : "kerr",
 				       1e3 * (td - it[k].arrival_s));
+				fflush(stdout);
+			}
 		}
 		if (!busy && next < n) {
 			double wait = it[next].arrival_s - (now_s() - t0);

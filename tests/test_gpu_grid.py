@@ -87,6 +87,40 @@ def test_render_fp64_frozen_bitwise(dtype):
     assert np.array_equal(i0, i1)
 
 
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 17), (9, 1), (13, 7)])
+def test_render_fp64_ragged_images_bitwise(w, h):
+    """Images that are not whole 8x8 tiles (a lone ray, a single column or
+    row, tiles cut on both sides) through the tile kernel: bit for bit the
+    oracle's."""
+    (d0, i0, s0), (d1, i1, s1) = render_pair(33, 8, (0.0, 1.0), w=w, h=h, t_init=1.0)
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0, d1), np.abs(d0 - d1).max()
+    assert np.array_equal(i0, i1)
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 17), (9, 1), (13, 7)])
+def test_render_fp32_ragged_images(w, h):
+    """The same shapes in the FP32 form against the plain FP64 form."""
+    n, times = 33, (0.0, 1.0)
+    xs, lv = make_grid(n, 4, times=times)
+    st = device_grid(n, 4, lv, times)
+    out = []
+    for prec in (G.FP64, G.FP32):
+        with G.Render(w, h, precision=prec, spacetime=st, t_init=1.0) as r:
+            r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+            r.render(-32.0, 1024, 2)
+            out.append((r.read_f64()[0], r.steps()))
+    st.close()
+    (d64, s64), (d32, s32) = out
+    assert d32.shape == d64.shape == (w * h, 8)
+    assert np.all(s32 > 0)
+    same = s64 == s32
+    if w * h >= 8:
+        assert same.mean() >= 0.5
+    if same.any():
+        assert np.abs(d32[same, 1:4] - d64[same, 1:4]).max() < 1e-2
+
+
 def test_render_fp64_two_levels_bitwise():
     (d0, i0, s0), (d1, i1, s1) = render_pair(33, 4, (0.0, 1.0), t_init=1.0)
     assert np.array_equal(s0, s1)
