@@ -152,6 +152,14 @@ __device__ __forceinline__ bool in_brick(const int c[3], const int o[3])
 	       (unsigned)(c[2] - o[2]) <= 2u;
 }
 
+/* A wave's brick state: origin point and evaluations left before the next
+ * reload may be tried.  Kept in registers by kernels whose lanes all take
+ * part in every RHS, so that every active lane holds the same values. */
+struct BrickState {
+	int o[3];
+	int cool;
+};
+
 /* Orders a wave's LDS writes before its later reads (and reads before
  * later writes): wave-scope fences around a scheduling barrier. */
 __device__ __forceinline__ void wave_sync()

@@ -32,11 +32,6 @@ namespace gray {
 
 constexpr int GT_WAVES = 2;                 /* waves per block */
 
-struct Brick64 {
-	int o[3];        /* origin point of the brick           */
-	int cool;        /* evaluations left without a reload   */
-};
-
 /* sample_lev_f64 with the corners in the brick (same sums, same order). */
 template <int NC, class S>
 __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, int base,
@@ -69,7 +64,7 @@ __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, in
  * evaluations before the next try.  Returns whether this lane's cell is in
  * the brick. */
 template <class S>
-__device__ __forceinline__ bool brick_update_f64(const GridView<S> &G, S *brick, Brick64 &b,
+__device__ __forceinline__ bool brick_update_f64(const GridView<S> &G, S *brick, BrickState &b,
                                                  const Stencil<double> &st)
 {
 	bool inside = in_brick(st.c, b.o);
@@ -110,7 +105,7 @@ __device__ __forceinline__ bool brick_update_f64(const GridView<S> &G, S *brick,
 
 template <int NF, class S>
 __device__ __forceinline__ void grid_rhs_brick_f64(const GridView<S> &G, const RtThermal &rt,
-                                                   S *brick, Brick64 &b, bool use_lds,
+                                                   S *brick, BrickState &b, bool use_lds,
                                                    const double *s, double *ds)
 {
 	Stencil<double> st;
@@ -158,7 +153,7 @@ __device__ __forceinline__ void grid_rhs_brick_f64(const GridView<S> &G, const R
 /* sim-org/RK4.cl:37-47 on the whole state (8 + 2 NF components). */
 template <int NF, class S>
 __device__ __forceinline__ void grid_rk4_brick_f64(const GridView<S> &G, const RtThermal &rt,
-                                                   S *brick, Brick64 &b, bool use_lds, double *s,
+                                                   S *brick, BrickState &b, bool use_lds, double *s,
                                                    double dt)
 {
 	constexpr int ND = 8 + 2 * NF;
@@ -204,7 +199,7 @@ __global__ __launch_bounds__(64 * GT_WAVES) void k_grid_tiles_f64(GridView<S> G,
 		return;
 	const size_t g = (size_t)y * (size_t)W + x;
 	S *brick = s_brick[wv];
-	Brick64 b = {{-1000000, -1000000, -1000000}, 0};   /* no cell matches: first RHS loads */
+	BrickState b = {{-1000000, -1000000, -1000000}, 0};   /* no cell matches: first RHS loads */
 	const bool use_lds = bricks && G.n[0] >= 4 && G.n[1] >= 4 && G.n[2] >= 4;
 
 	double s[ND];
