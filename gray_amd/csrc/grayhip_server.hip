@@ -377,26 +377,42 @@ static int admit(grayhip_server *v)
 	return GRAYHIP_OK;
 }
 
-/* Give free slots to rays of admitted requests, oldest request first. */
+/* Give free slots to rays of admitted requests.  The free slots are shared
+ * evenly among the requests still waiting for slots (the oldest take the
+ * remainder), so a small request that arrives behind a large one gets its
+ * slots at the next epoch instead of after the large one's last ray. */
 static int refill(grayhip_server *v, int *n_assign)
 {
 	const size_t S = (size_t)v->S.S;
 	int n = 0;
 	int *as = v->h_assign, *ar = v->h_assign + S, *ay = v->h_assign + 2 * S;
 	while (!v->filling.empty() && !v->free_slots.empty()) {
-		HReq &r = v->reqs[v->filling.front()];
-		while (r.next < r.n_rays && !v->free_slots.empty()) {
-			int s = v->free_slots.back();
-			v->free_slots.pop_back();
-			as[n] = s;
-			ar[n] = r.tab;
-			ay[n] = (int)r.next++;
-			v->slot_req[s] = r.id;
-			list_add(v, r.kind == SRV_GRID ? 2 : 0, s);
-			++n;
+		const size_t n_req = v->filling.size();
+		const size_t share = v->free_slots.size() / n_req;
+		size_t extra = v->free_slots.size() % n_req;
+		for (size_t k = 0; k < n_req && !v->free_slots.empty(); ++k) {
+			HReq &r = v->reqs[v->filling[k]];
+			size_t give = share + (extra > 0 ? 1 : 0);
+			if (extra > 0)
+				--extra;
+			for (; give > 0 && r.next < r.n_rays && !v->free_slots.empty(); --give) {
+				int s = v->free_slots.back();
+				v->free_slots.pop_back();
+				as[n] = s;
+				ar[n] = r.tab;
+				ay[n] = (int)r.next++;
+				v->slot_req[s] = r.id;
+				list_add(v, r.kind == SRV_GRID ? 2 : 0, s);
+				++n;
+			}
 		}
-		if (r.next == r.n_rays)
-			v->filling.pop_front();
+		/* drop requests whose rays all have slots; slots a full request
+		 * left unused go round again */
+		std::deque<int64_t> still;
+		for (int64_t id : v->filling)
+			if (v->reqs[id].next < v->reqs[id].n_rays)
+				still.push_back(id);
+		v->filling.swap(still);
 	}
 	*n_assign = n;
 	if (n) {
