@@ -92,7 +92,7 @@ def headline(G, args, out):
         r, _, _ = render(G, G.FP32)
         r.close()
     walls, total_steps, kern_ns, pool_steps = [], 0, 0.0, 0
-    img32 = None
+    img32 = info32 = None
     t_all = time.perf_counter()
     for k in range(args.steps):
         r, wall, ns = render(G, G.FP32)
@@ -104,7 +104,7 @@ def headline(G, args, out):
         kern_ns += kn
         pool_steps += ps
         if k == args.steps - 1:
-            d32, _ = r.read_f64()
+            d32, info32 = r.read_f64()
             img32 = d32[:, 8:12].copy()
             out["fallback_rays"] = nfb
             out["fallback_ray_steps"] = fbs
@@ -130,13 +130,13 @@ def headline(G, args, out):
         "flop_per_ray_step": FLOP_PER_STEP,
         "kernel_ms": 1e3 * k_s,
     }
-    return img32, value
+    return img32, info32, value
 
 
-def fp64_section(G, img32, value, out):
+def fp64_section(G, img32, info32, value, out):
     r, wall, ns = render(G, G.FP64)
     st64, _ = r.stats()
-    d64, _ = r.read_f64()
+    d64, info64 = r.read_f64()
     kn64, nl = r.kernel_time()
     r.close()
     img64 = d64[:, 8:12]
@@ -151,6 +151,11 @@ def fp64_section(G, img32, value, out):
     out["speedup_fp32_vs_fp64"] = value / fp64_rate
     out["image_rel_l1_vs_fp64"] = l1(img32, img64)
     out["image_peak_err_vs_fp64"] = float(np.abs(img32 - img64).max() / np.abs(img64).max())
+    # null constraint |g(k,k)| at the end, over the rays that escaped (r > 999 M)
+    esc = np.sqrt(np.sum(d64[:, 1:4] ** 2, axis=1)) > 999.0
+    out["null_drift_max"] = {"fp32": float(np.abs(info32[esc]).max()),
+                             "fp64": float(np.abs(info64[esc]).max()),
+                             "rays": int(esc.sum())}
     return fp64_rate
 
 
@@ -343,10 +348,10 @@ def main():
             "rays": W * H, "n_freq": len(NUS), "form": "fp32 pool + fp64 fallback",
         },
     }
-    img32, value = headline(G, args, out)
+    img32, info32, value = headline(G, args, out)
     log(f"headline: {value:.3e} ray-steps/s")
     if not args.no_fp64:
-        fp64_section(G, img32, value, out)
+        fp64_section(G, img32, info32, value, out)
         log(f"fp64: {out['fp64']['value']:.3e} ray-steps/s")
     if not args.no_grid:
         grid_section(G, out)
