@@ -19,6 +19,9 @@
 
 using namespace gray;
 
+/* Pool passes of the FP32 form over bands of image rows (run_fp32). */
+constexpr int FP32_PASSES = 3;
+
 struct grayhip_ctx {
 	grayhip_config cfg;
 	RayParams p;
@@ -30,8 +33,8 @@ struct grayhip_ctx {
 	Counters *d_ctr;      /* counters of the FP32 pass                       */
 	Fallback *d_fb;
 	void *d_tmp;          /* AoS staging for read/write                      */
-	hipStream_t stream, stream2;
-	hipEvent_t ev[8];
+	hipStream_t stream, stream2, stream3;
+	hipEvent_t ev[12];
 	bool have_icond;
 	int pool_blocks;
 	grayhip_spacetime *st;  /* numerical spacetime, or NULL (analytic Kerr) */
@@ -127,13 +130,15 @@ static void release(grayhip_ctx *c)
 	(void)hipFree(c->d_ctr);
 	(void)hipFree(c->d_fb);
 	(void)hipFree(c->d_tmp);
-	for (int i = 0; i < 8; ++i)
+	for (int i = 0; i < 12; ++i)
 		if (c->ev[i])
 			(void)hipEventDestroy(c->ev[i]);
 	if (c->stream)
 		(void)hipStreamDestroy(c->stream);
 	if (c->stream2)
 		(void)hipStreamDestroy(c->stream2);
+	if (c->stream3)
+		(void)hipStreamDestroy(c->stream3);
 	free(c);
 }
 
@@ -222,7 +227,7 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 			{(void **)&c->d_data, n * nd * sizeof(double)},
 			{(void **)&c->d_info, n * sizeof(double)},
 			{(void **)&c->d_steps, n * sizeof(int64_t)},
-			{(void **)&c->d_ctr, 2 * sizeof(Counters)},
+			{(void **)&c->d_ctr, FP32_PASSES * sizeof(Counters)},
 			{(void **)&c->d_fb, n * sizeof(Fallback)},
 			{(void **)&c->d_tmp, n * nd * sizeof(double)},
 		};
@@ -235,11 +240,12 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 		}
 	}
 	if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
-	    (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess) {
+	    (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
+	    (e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking)) != hipSuccess) {
 		rc = fail(nullptr, GRAYHIP_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
 		goto bad;
 	}
-	for (int i = 0; i < 8; ++i)
+	for (int i = 0; i < 12; ++i)
 		if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) {
 			rc = fail(nullptr, GRAYHIP_EDEVICE, "hipEventCreate: %s", hipGetErrorString(e));
 			goto bad;
@@ -360,63 +366,106 @@ static int make_rule(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, Ste
 	return GRAYHIP_OK;
 }
 
-/* FP32 form: two persistent pool passes -- the middle half of the image
- * rows (the shadow, whose rays reach the horizon and collapse their step)
- * and then the outer rows -- with the FP64 re-integration of the first
- * pass's hand-overs on a second stream, beside the second pass; the second
- * pass's few hand-overs follow it.  Fallback launches cover the largest
- * possible queue and idle threads exit, so nothing waits on the host. */
+/* FP32 form: persistent pool passes over bands of image rows, nearest the
+ * centre row first, each followed at once by the FP64 re-integration of its
+ * hand-overs on a stream of its own, beside the later passes.  The bands
+ * are |beta| <= 2 M, which holds the rays that wind closest to a prograde
+ * photon orbit (at a = 0.998, i = 85 up to 1e6 steps, near beta = +-0.9,
+ * alpha = -2.1), so their long FP64 chains start after a short pass; then
+ * |beta| <= 8 M, the rest of the photon ring and the shadow edge for every
+ * spin and inclination (critical impact parameters 2-7 M); then the outer
+ * rows, which hand over few rays.  A band with fewer rays than the pool has
+ * lanes would leave the GPU part idle for its whole pass, so it joins the
+ * next band instead (at 1024^2 the first two bands are one pass).
+ * Fallback launches cover the largest possible queue and idle threads
+ * exit, so nothing waits on the host. */
 static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 {
 	const int W = (int)c->cfg.w_rays, H = (int)c->cfg.h_rays;
-	/* Pass 1: the rows within 8 M of the image centre, where the photon
-	 * ring and the shadow edge lie for every spin and inclination (critical
-	 * impact parameters 2-7 M); the rays whose steps collapse come from
-	 * there, so their FP64 re-integration overlaps pass 2. */
-	const double band = 8.0 * (double)H / fabs(c->p.h_img > 0.0 ? c->p.h_img : 1.0);
-	int r0 = (int)floor(0.5 * H - band), r1 = (int)ceil(0.5 * H + band);
-	r0 = r0 < 0 ? 0 : r0;
-	r1 = r1 > H ? H : r1;
-	RayRange mid = {r0 * W, r1 * W, 0, 0};
-	RayRange out = {0, r0 * W, r1 * W, H * W};
-	const unsigned n_mid = (unsigned)((r1 - r0) * W), n_out = (unsigned)(c->n_rays - n_mid);
-	Fallback *fb_mid = c->d_fb, *fb_out = c->d_fb + n_mid;
-	hipStream_t A = c->stream, B = c->stream2;
-
-	HCK(c, hipMemsetAsync(c->d_ctr, 0, 2 * sizeof(Counters), A));
+	const double px = (double)H / fabs(c->p.h_img > 0.0 ? c->p.h_img : 1.0);
+	const double band[FP32_PASSES - 1] = {2.0, 8.0};
+	const size_t lanes = (size_t)c->pool_blocks * 256;
+	/* row bands [lo, hi) of the passes before the outer one, innermost first */
+	int lo[FP32_PASSES - 1], hi[FP32_PASSES - 1], n_in = 0;
+	int plo = (H + 1) / 2, phi = (H + 1) / 2;   /* rows already taken: none */
+	for (int k = 0; k < FP32_PASSES - 1; ++k) {
+		int a = (int)floor(0.5 * H - band[k] * px), b = (int)ceil(0.5 * H + band[k] * px);
+		a = a < 0 ? 0 : a;
+		b = b > H ? H : b;
+		const size_t rays = (size_t)((plo - a) + (b - phi)) * (size_t)W;
+		if (a >= plo && b <= phi)
+			continue;
+		if (rays < lanes)
+			continue;   /* too few rays for a pass of its own: join the next band */
+		lo[n_in] = a;
+		hi[n_in] = b;
+		++n_in;
+		plo = a;
+		phi = b;
+	}
+	const int n_pass = n_in + 1;
+	RayRange rr[FP32_PASSES];
+	for (int k = 0; k < n_in; ++k) {
+		const int ilo = k ? lo[k - 1] : (H + 1) / 2, ihi = k ? hi[k - 1] : (H + 1) / 2;
+		rr[k] = {lo[k] * W, ilo * W, ihi * W, hi[k] * W};
+	}
+	rr[n_in] = {0, plo * W, phi * W, H * W};
+	unsigned cnt[FP32_PASSES];
+	Fallback *fb[FP32_PASSES];
+	size_t off = 0;
+	for (int k = 0; k < n_pass; ++k) {
+		cnt[k] = (unsigned)((rr[k].e0 - rr[k].b0) + (rr[k].e1 - rr[k].b1));
+		fb[k] = c->d_fb + off;
+		off += cnt[k];
+	}
+	if (off != c->n_rays)
+		return fail(c, GRAYHIP_EDEVICE, "pass ranges do not cover the image");
+	hipStream_t A = c->stream;
+	/* the last pass's hand-overs follow it on its own stream */
+	hipStream_t fs[FP32_PASSES] = {c->stream2, c->stream3, c->stream};
+	fs[n_pass - 1] = A;
+	/* events: 0/1 whole render, 2+2k / 3+2k pass k, 8+k fallback k done */
+	HCK(c, hipMemsetAsync(c->d_ctr, 0, FP32_PASSES * sizeof(Counters), A));
 	HCK(c, hipEventRecord(c->ev[0], A));
-	HCK(c, hipEventRecord(c->ev[2], A));
-	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, mid, c->d_ctr,
-	                       fb_mid, (unsigned)c->pool_blocks, A));
-	HCK(c, hipEventRecord(c->ev[3], A));
-	HCK(c, hipStreamWaitEvent(B, c->ev[3], 0));
-	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb_mid,
-	                           c->d_ctr, n_mid, B));
-	HCK(c, hipEventRecord(c->ev[6], B));
-	HCK(c, hipEventRecord(c->ev[4], A));
-	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, out,
-	                       c->d_ctr + 1, fb_out, (unsigned)c->pool_blocks, A));
-	HCK(c, hipEventRecord(c->ev[5], A));
-	HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb_out,
-	                           c->d_ctr + 1, n_out, A));
-	HCK(c, hipStreamWaitEvent(A, c->ev[6], 0));
+	for (int k = 0; k < n_pass; ++k) {
+		HCK(c, hipEventRecord(c->ev[2 + 2 * k], A));
+		HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, rr[k],
+		                       c->d_ctr + k, fb[k], (unsigned)c->pool_blocks, A));
+		HCK(c, hipEventRecord(c->ev[3 + 2 * k], A));
+		if (fs[k] != A)
+			HCK(c, hipStreamWaitEvent(fs[k], c->ev[3 + 2 * k], 0));
+		HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb[k],
+		                           c->d_ctr + k, cnt[k], fs[k]));
+		if (fs[k] != A)
+			HCK(c, hipEventRecord(c->ev[8 + k], fs[k]));
+	}
+	for (int k = 0; k < n_pass; ++k)
+		if (fs[k] != A)
+			HCK(c, hipStreamWaitEvent(A, c->ev[8 + k], 0));
 	HCK(c, hipEventRecord(c->ev[1], A));
-	Counters h[2];
-	HCK(c, hipMemcpyAsync(h, c->d_ctr, 2 * sizeof(Counters), hipMemcpyDeviceToHost, A));
+	Counters h[FP32_PASSES];
+	HCK(c, hipMemcpyAsync(h, c->d_ctr, n_pass * sizeof(Counters), hipMemcpyDeviceToHost, A));
 	HCK(c, hipStreamSynchronize(A));
-	if (h[0].n_fb > n_mid || h[1].n_fb > n_out)
-		return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow");
-	float ms = 0.f, k0 = 0.f, k1 = 0.f;
+	for (int k = 0; k < n_pass; ++k)
+		if (h[k].n_fb > cnt[k])
+			return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow");
+	float ms = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-	HCK(c, hipEventElapsedTime(&k0, c->ev[2], c->ev[3]));
-	HCK(c, hipEventElapsedTime(&k1, c->ev[4], c->ev[5]));
-	c->kern_ns = 1e6 * ((double)k0 + (double)k1);
-	c->kern_launches = (n_mid > 0) + (n_out > 0);
-	c->last_fallback = (int64_t)h[0].n_fb + h[1].n_fb;
-	c->last_ctr = h[0];
-	c->last_ctr.n_fb += h[1].n_fb;
-	c->last_ctr.pool_steps += h[1].pool_steps;
-	c->last_ctr.fb_steps += h[1].fb_steps;
+	double kern = 0.0;
+	c->kern_launches = 0;
+	c->last_fallback = 0;
+	memset(&c->last_ctr, 0, sizeof(c->last_ctr));
+	for (int k = 0; k < n_pass; ++k) {
+		float kk = 0.f;
+		HCK(c, hipEventElapsedTime(&kk, c->ev[2 + 2 * k], c->ev[3 + 2 * k]));
+		kern += (double)kk;
+		c->kern_launches += cnt[k] > 0;
+		c->last_fallback += (int64_t)h[k].n_fb;
+		c->last_ctr.n_fb += h[k].n_fb;
+		c->last_ctr.pool_steps += h[k].pool_steps;
+		c->last_ctr.fb_steps += h[k].fb_steps;
+	}
+	c->kern_ns = 1e6 * kern;
 	if (ns)
 		*ns = 1e6 * (double)ms;
 	return GRAYHIP_OK;
