@@ -11,9 +11,6 @@
 
 namespace gray {
 
-template <class S>
-__device__ __forceinline__ double ld(const S *a, uint32_t i) { return (double)a[i]; }
-
 /* Fields base..base+NC-1 of one level (interleaved, stride NCOMP) summed over
  * the 8 corners in corner order: s = w0 v0; s = s + wk vk. */
 template <int NC, int NCOMP, class S>
