@@ -127,7 +127,7 @@ __device__ __forceinline__ void gr_icond(T a, T r_obs, T si, T ci, T sj, T cj,
  * g = eta + f l l, so k_mu = eta_mu nu u^nu + f (l.u) l_mu needs no second
  * evaluation (used by the FP32 form's transfer). */
 template <class T> struct KsAux {
-	T r, f, lx, ly, lz, lu;
+	T r, ir, f, lx, ly, lz, lu;   /* ir = 1 / r */
 };
 
 /* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS.
@@ -146,22 +146,42 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 		T aa = a * a;
 		T zz = q.z * q.z;
 		T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
-		T dd = M::sqrt_(kk * kk + aa * zz);
-		T rr = dd + kk;
-		T r  = M::sqrt_(rr);
-		if (aux)
+		T dd, rr, r, ir, half_rdd, two_rq;
+		if constexpr (sizeof(T) == 4) {
+			/* FP32 form: the same quantities from two reciprocal square
+			 * roots and one reciprocal (transcendentals issue at a quarter
+			 * of the VALU rate) instead of two square roots and four
+			 * reciprocals; 2 / (rr + aa zz / rr) = 2 rr / (rr^2 + aa zz). */
+			const T d2 = kk * kk + aa * zz;
+			const T idd = __builtin_amdgcn_rsqf(d2);
+			dd = d2 * idd;
+			rr = dd + kk;
+			ir = __builtin_amdgcn_rsqf(rr);
+			r = rr * ir;
+			half_rdd = (T(0.5) * ir) * idd;
+			two_rq = M::div(T(2.0) * rr, rr * rr + aa * zz);
+		} else {
+			dd = M::sqrt_(kk * kk + aa * zz);
+			rr = dd + kk;
+			r  = M::sqrt_(rr);
+			ir = M::div(T(1.0), r);
+			half_rdd = M::div(T(0.5), r * dd);
+			two_rq = M::div(T(2.0), rr + M::div(aa * zz, rr));
+		}
+		if (aux) {
 			aux->r = r;
-		T ir = M::div(T(1.0), r);
+			aux->ir = ir;
+		}
 		T ss = rr + aa;
 		T iss = M::div(T(1.0), ss);
 		T dx_r, dy_r, dz_r, tmp2;
 
-		tmp  = M::div(T(0.5), r * dd);
+		tmp  = half_rdd;
 		dz_r = (tmp * ss) * q.z;
 		tmp  = tmp * rr;
 		dy_r = tmp * q.y;
 		dx_r = tmp * q.x;
-		tmp  = M::div(T(2.0), rr + M::div(aa * zz, rr));
+		tmp  = two_rq;
 
 		tmp2 = T(3.0) - (T(2.0) * rr) * tmp;
 		f    = tmp * r;

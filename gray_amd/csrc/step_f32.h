@@ -45,7 +45,9 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 	if (p.fluid == GRAYHIP_FLUID_DISK) {
 		const float R2 = q.x * q.x + q.y * q.y;
 		if (K.r >= (float)p.r_isco && R2 > 0.f) {
-			const float rs = K.r * M::sqrt_(K.r);
+			/* r^(3/2) and r^(-3/2) from one reciprocal square root */
+			const float isr = __builtin_amdgcn_rsqf(K.r);
+			const float rs = (K.r * K.r) * isr;
 			const float Om = M::div(1.f, rs + (float)p.a);
 			const float lv = 1.f + Om * (q.x * K.ly - q.y * K.lx);
 			const float vv = (Om * Om * R2 - 1.f) + K.f * lv * lv;
@@ -57,7 +59,7 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 				const float s_ = -kv * ut;
 				if (s_ > 0.f) {
 					sh = s_;
-					n = M::div(1.f, rs) * M::exp_(M::div(-(q.z * q.z), 0.18f * R2));
+					n = (K.ir * isr) * M::exp_(M::div(-(q.z * q.z), 0.18f * R2));
 				}
 			}
 		}
