@@ -375,16 +375,19 @@ static int make_rule(grayhip_ctx *c, double dt, size_t n_sub, size_t n_dump, Ste
  * |beta| <= 8 M, the rest of the photon ring and the shadow edge for every
  * spin and inclination (critical impact parameters 2-7 M); then the outer
  * rows, which hand over few rays.  A band with fewer rays than the pool has
- * lanes would leave the GPU part idle for its whole pass, so it joins the
- * next band instead (at 1024^2 the first two bands are one pass).
+ * lanes would leave the GPU part idle for its whole pass: the first band is
+ * widened to the rows that fill the lanes (|beta| <= 6 M at 1024^2), and a
+ * later band that would still be short joins the next one.
  * Fallback launches cover the largest possible queue and idle threads
  * exit, so nothing waits on the host. */
 static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 {
 	const int W = (int)c->cfg.w_rays, H = (int)c->cfg.h_rays;
 	const double px = (double)H / fabs(c->p.h_img > 0.0 ? c->p.h_img : 1.0);
-	const double band[FP32_PASSES - 1] = {2.0, 8.0};
 	const size_t lanes = (size_t)c->pool_blocks * 256;
+	/* the first band is widened to the rows that fill the pool's lanes */
+	const double fill = 0.5 * ceil((double)lanes / (double)W) / px;
+	const double band[FP32_PASSES - 1] = {fill > 2.0 ? fill : 2.0, 8.0};
 	/* row bands [lo, hi) of the passes before the outer one, innermost first */
 	int lo[FP32_PASSES - 1], hi[FP32_PASSES - 1], n_in = 0;
 	int plo = (H + 1) / 2, phi = (H + 1) / 2;   /* rows already taken: none */
