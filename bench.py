@@ -328,8 +328,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (rays shard across GPUs with no "
-              "exchange; multi-GPU runs are not part of it)", file=sys.stderr)
+        print("bench.py: runs on one GPU only (rays would shard across GPUs with no "
+              "exchange; multi-GPU runs are not implemented)", file=sys.stderr)
         sys.exit(2)
 
     from gray_amd import grayhip as G
@@ -353,28 +353,45 @@ def main():
     if not args.no_fp64:
         fp64_section(G, img32, info32, value, out)
         log(f"fp64: {out['fp64']['value']:.3e} ray-steps/s")
+    def optional(name, fn):
+        # a failing side section is reported in the line instead of losing it
+        try:
+            fn()
+        except Exception as e:  # noqa: BLE001
+            out[f"{name}_error"] = repr(e)
+            log(f"{name} failed: {e!r}")
+
     if not args.no_grid:
-        grid_section(G, out)
-        log(f"grid: fp32 {out['grid']['fp32_grid']['value']:.3e} ray-steps/s")
+        def grid():
+            grid_section(G, out)
+            log(f"grid: fp32 {out['grid']['fp32_grid']['value']:.3e} ray-steps/s")
+        optional("grid", grid)
     if not args.no_serve:
-        if "fp64" in out:
-            serving_section(G, args, out["fp64"]["value"], "plain-FP64", out)
-        else:
-            serving_section(G, args, value, "FP32-form", out)
-        log(f"serving: p50 {out['serving']['p50_ms']:.0f} ms, p99 {out['serving']['p99_ms']:.0f} ms")
+        def serve():
+            if "fp64" in out:
+                serving_section(G, args, out["fp64"]["value"], "plain-FP64", out)
+            else:
+                serving_section(G, args, value, "FP32-form", out)
+            log(f"serving: p50 {out['serving']['p50_ms']:.0f} ms, "
+                f"p99 {out['serving']['p99_ms']:.0f} ms")
+        optional("serving", serve)
     if args.precision_form:
-        precision_section(G, out)
-        log("precision form done")
+        def prec():
+            precision_section(G, out)
+            log("precision form done")
+        optional("precision_form", prec)
     if not args.no_cpu:
-        n_thr = min(os.cpu_count() or 1, 16)
-        rate, steps, dt, n = cpu_baseline(args.cpu_stride, n_thr)
-        out["cpu_baseline"] = {
-            "value": rate, "unit": "ray-steps/s", "cores": n_thr, "kind": "port",
-            "sample": f"FP64 oracle (CPU restatement of the reference FP64 path, C, "
-                      f"-O3 -ffp-contract=off, OpenMP over rays) on every {args.cpu_stride}th "
-                      f"row and column ({n} rays, {steps} ray-steps, {dt:.1f} s) on "
-                      f"{n_thr} threads of {cpu_model()}",
-        }
+        def cpu():
+            n_thr = min(os.cpu_count() or 1, 16)
+            rate, steps, dt, n = cpu_baseline(args.cpu_stride, n_thr)
+            out["cpu_baseline"] = {
+                "value": rate, "unit": "ray-steps/s", "cores": n_thr, "kind": "port",
+                "sample": f"FP64 oracle (CPU restatement of the reference FP64 path, C, "
+                          f"-O3 -ffp-contract=off, OpenMP over rays) on every {args.cpu_stride}th "
+                          f"row and column ({n} rays, {steps} ray-steps, {dt:.1f} s) on "
+                          f"{n_thr} threads of {cpu_model()}",
+            }
+        optional("cpu_baseline", cpu)
     print(json.dumps(out), flush=True)
 
 
