@@ -131,12 +131,12 @@ template <class T> struct KsAux {
 };
 
 /* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS.
- * Returns du/dlambda (dq/dlambda = u); fills *aux when given. */
-template <class T>
+ * Returns du/dlambda (dq/dlambda = u); fills *aux when given.  M supplies
+ * sqrt and division (the correctly rounded ones by default). */
+template <class T, class M = Mth<T>>
 __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
                                         KsAux<T> *aux = nullptr)
 {
-	typedef Mth<T> M;
 	T f, dx_f, dy_f, dz_f;
 	T lx, dx_lx, dy_lx, dz_lx;
 	T ly, dx_ly, dy_ly, dz_ly;

@@ -2,11 +2,12 @@
  * kernels_fb.hip -- FP64 re-integration of the rays the FP32 pass hands
  * over (step collapse near the horizon, or a non-finite FP32 state).
  *
- * Same FP64 step code as the plain form (step_f64.h) but compiled with
+ * The FP64 step of step_fb.h (few instructions per step: these rays'
+ * sequential steps are a render's critical path), compiled with
  * multiply-add contraction: these rays are not part of the bit-exact form.
  */
 #include "launch.h"
-#include "step_f64.h"
+#include "step_fb.h"
 
 namespace gray {
 
@@ -30,6 +31,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 	 * longest ray is the render's critical path: raise their issue
 	 * priority so the pool waves fill the gaps of their dependency chain. */
 	__builtin_amdgcn_s_setprio(3);
+	const RtConstFb<NF> rc = make_rtconst_fb<NF>(p);
 	Fallback f = fb[e];
 	size_t g = (size_t)f.g;
 	double s[ND];
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 	int64_t st = st0;
 	for (int d = d0; d < sr.n_dump; ++d) {
 		bool retired;
-		st += interval_f64<NF>(p, sr, s, d == d0 ? h0 : 0, &retired);
+		st += interval_fb<NF>(p, rc, sr, s, d == d0 ? h0 : 0, &retired);
 		if (retired)
 			break;
 	}

@@ -10,6 +10,7 @@
  *   k_srv_init    camera initial conditions in FP64 for newly assigned slots
  *   k_srv_kerr32  analytic Kerr, FP32 form (per-lane spin, channels, camera)
  *   k_srv_kerr64  analytic Kerr, FP64 re-integration of collapsed-step rays
+ *                 (the step of step_fb.h, as the render's re-integration)
  *   k_srv_grid32  gridded spacetimes, FP32 form
  *
  * Built with multiply-add contraction (these are FP32-form kernels).
@@ -21,6 +22,7 @@
 #include "launch_serve.h"
 #include "step_f32.h"
 #include "step_f64.h"
+#include "step_fb.h"
 
 namespace gray {
 
@@ -221,6 +223,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 	const int s = list[idx];
 	const SrvRequest &R = reqs[S.req[s]];
 	const StepRule sr = R.sr;
+	const RtConstFb<SRV_NF> rc = make_rtconst_fb<SRV_NF>(R.p);
 	double x[SRV_NC];
 	int d = S.d[s], h = S.h[s];
 	int64_t st = S.steps[s];
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 		}
 		if (dh > sr.n - h)
 			dh = sr.n - h;
-		rk4_f64<SRV_NF, RayParams>(R.p, x, (double)dh * sr.dt / (double)sr.n);
+		rk4_fb<SRV_NF, RayParams>(R.p, rc, x, (double)dh * sr.dt / (double)sr.n);
 		++st;
 		h += dh;
 		if (h >= sr.n) {
