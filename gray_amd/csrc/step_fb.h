@@ -55,7 +55,9 @@ __device__ __forceinline__ RtConstFb<NF> make_rtconst_fb(const P &p)
 	RtConstFb<NF> r;
 #pragma unroll
 	for (int i = 0; i < (NF > 0 ? NF : 1); ++i) {
-		const double nu = NF > 0 ? (double)p.nu[i] : 1.0;
+		/* a serving slot carries NF channels whatever its request asked
+		 * for: the unused ones repeat channel 0 (their results are dropped) */
+		const double nu = NF > 0 ? (double)p.nu[i < p.n_freq ? i : 0] : 1.0;
 		r.c[i] = 1.0 / sqrt(nu);
 		r.d[i] = 0.1 / (nu * nu * sqrt(nu));
 	}
