@@ -423,28 +423,46 @@ static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 	}
 	if (off != c->n_rays)
 		return fail(c, GRAYHIP_EDEVICE, "pass ranges do not cover the image");
-	hipStream_t A = c->stream;
-	/* the last pass's hand-overs follow it on its own stream */
-	hipStream_t fs[FP32_PASSES] = {c->stream2, c->stream3, c->stream};
-	fs[n_pass - 1] = A;
-	/* events: 0/1 whole render, 2+2k / 3+2k pass k, 8+k fallback k done */
+	/* Pass 0 runs on A; the later passes run in order on C beside it, so
+	 * their blocks take the slots pass 0's finished blocks leave (pass 0
+	 * has about one ray per lane and a tail of long rays).  Pass 0's
+	 * hand-overs are re-integrated on B as soon as it ends, a middle pass's
+	 * on A, the last pass's on C after it.
+	 * events: 0/1 render, 2+2k / 3+2k pass k, 8 B done, 9 C done, 11 ready */
+	hipStream_t A = c->stream, B = c->stream2, C = c->stream3;
 	HCK(c, hipMemsetAsync(c->d_ctr, 0, FP32_PASSES * sizeof(Counters), A));
 	HCK(c, hipEventRecord(c->ev[0], A));
-	for (int k = 0; k < n_pass; ++k) {
-		HCK(c, hipEventRecord(c->ev[2 + 2 * k], A));
-		HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, rr[k],
-		                       c->d_ctr + k, fb[k], (unsigned)c->pool_blocks, A));
-		HCK(c, hipEventRecord(c->ev[3 + 2 * k], A));
-		if (fs[k] != A)
-			HCK(c, hipStreamWaitEvent(fs[k], c->ev[3 + 2 * k], 0));
-		HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb[k],
-		                           c->d_ctr + k, cnt[k], fs[k]));
-		if (fs[k] != A)
-			HCK(c, hipEventRecord(c->ev[8 + k], fs[k]));
+	HCK(c, hipEventRecord(c->ev[11], A));
+	HCK(c, hipEventRecord(c->ev[2], A));
+	HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, rr[0],
+	                       c->d_ctr, fb[0], (unsigned)c->pool_blocks, A));
+	HCK(c, hipEventRecord(c->ev[3], A));
+	if (n_pass == 1) {
+		HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb[0],
+		                           c->d_ctr, cnt[0], A));
+	} else {
+		HCK(c, hipStreamWaitEvent(B, c->ev[3], 0));
+		HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, fb[0],
+		                           c->d_ctr, cnt[0], B));
+		HCK(c, hipEventRecord(c->ev[8], B));
+		HCK(c, hipStreamWaitEvent(C, c->ev[11], 0));
+		for (int k = 1; k < n_pass; ++k) {
+			HCK(c, hipEventRecord(c->ev[2 + 2 * k], C));
+			HCK(c, launch_pool_f32(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays, rr[k],
+			                       c->d_ctr + k, fb[k], (unsigned)c->pool_blocks, C));
+			HCK(c, hipEventRecord(c->ev[3 + 2 * k], C));
+			hipStream_t f = C;
+			if (k < n_pass - 1) {
+				f = A;
+				HCK(c, hipStreamWaitEvent(A, c->ev[3 + 2 * k], 0));
+			}
+			HCK(c, launch_fallback_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays,
+			                           fb[k], c->d_ctr + k, cnt[k], f));
+		}
+		HCK(c, hipEventRecord(c->ev[9], C));
+		HCK(c, hipStreamWaitEvent(A, c->ev[8], 0));
+		HCK(c, hipStreamWaitEvent(A, c->ev[9], 0));
 	}
-	for (int k = 0; k < n_pass; ++k)
-		if (fs[k] != A)
-			HCK(c, hipStreamWaitEvent(A, c->ev[8 + k], 0));
 	HCK(c, hipEventRecord(c->ev[1], A));
 	Counters h[FP32_PASSES];
 	HCK(c, hipMemcpyAsync(h, c->d_ctr, n_pass * sizeof(Counters), hipMemcpyDeviceToHost, A));
@@ -454,14 +472,16 @@ static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 			return fail(c, GRAYHIP_EDEVICE, "fallback queue overflow");
 	float ms = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+	/* pool time: from the first pass's start to the last pass's end (the
+	 * passes overlap) */
 	double kern = 0.0;
 	c->kern_launches = 0;
 	c->last_fallback = 0;
 	memset(&c->last_ctr, 0, sizeof(c->last_ctr));
 	for (int k = 0; k < n_pass; ++k) {
 		float kk = 0.f;
-		HCK(c, hipEventElapsedTime(&kk, c->ev[2 + 2 * k], c->ev[3 + 2 * k]));
-		kern += (double)kk;
+		HCK(c, hipEventElapsedTime(&kk, c->ev[2], c->ev[3 + 2 * k]));
+		kern = (double)kk > kern ? (double)kk : kern;
 		c->kern_launches += cnt[k] > 0;
 		c->last_fallback += (int64_t)h[k].n_fb;
 		c->last_ctr.n_fb += h[k].n_fb;
