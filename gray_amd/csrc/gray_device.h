@@ -285,74 +285,4 @@ __device__ __forceinline__ T getdt(const P &p, const V4<T> &q,
 	return dt;
 }
 
-/* Synthetic disk: n = r^-1.5 exp(-z^2/(2 (0.3 R)^2)) outside r_isco, zero
- * on the axis or where the Keplerian velocity is not timelike.  Keplerian
- * Omega = 1/(r^1.5 + a); v = (1, -y Omega, x Omega, 0), u = v/sqrt(-g(v,v));
- * shift = -k.u with k = down(q, u_photon)  (sim-org/flow.cl:36-58 with the
- * fluid model replaced).  Inside r_isco nothing else is evaluated, so the
- * horizon approach pays for the geodesic only. */
-template <class T, class P>
-__device__ __forceinline__ void flow_disk(const P &p, const V4<T> &q,
-                                          const V4<T> &u, T &n, T &shift)
-{
-	typedef Mth<T> M;
-	T a  = T(p.a);
-	T r  = M::sqrt_(getrr(a, q));
-	T R2 = q.x * q.x + q.y * q.y;
-
-	n = T(0.0);
-	shift = T(1.0);
-	if (!(r >= T(p.r_isco)) || !(R2 > T(0.0)))
-		return;
-	V4<T> k = down(a, q, u);
-	T rs = r * M::sqrt_(r);
-	T Om = M::div(T(1.0), rs + a);
-	V4<T> v = {T(1.0), -(q.y * Om), q.x * Om, T(0.0)};
-	T vv = getuu(a, q, v);
-	if (!(vv < T(0.0)))
-		return;
-	T ut = M::div(T(1.0), M::sqrt_(-vv));
-	T sh = -(dot4(k, v) * ut);
-	if (!(sh > T(0.0)))
-		return;
-	shift = sh;
-	n = M::div(T(1.0), rs) * M::exp_(M::div(-(q.z * q.z), T(0.18) * R2));
-}
-
-/* sim-org/rt.cl:185-198 transfer structure with the synthetic emissivity
- * j = n nu^-1/2 and absorption alpha = 0.1 n nu^-5/2 at nu = nu_i shift:
- *   dI/dl = -j exp(-tau) / (shift^2 + 1e-28),  dtau/dl = -alpha shift.
- * Fills dI[0..NF), dtau[0..NF). */
-template <class T, int NF, class P>
-__device__ __forceinline__ void rt_rhs(const P &p, const V4<T> &q,
-                                       const V4<T> &u, const T *tau,
-                                       T *dI, T *dtau)
-{
-	typedef Mth<T> M;
-	T n = T(0.0), shift = T(1.0);
-	if (p.fluid == GRAYHIP_FLUID_DISK) {
-		flow_disk(p, q, u, n, shift);
-	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
-		n = T(1.0);
-	}
-	if (n == T(0.0)) {
-#pragma unroll
-		for (int i = 0; i < NF; ++i) {
-			dI[i] = T(0.0);
-			dtau[i] = T(0.0);
-		}
-		return;
-	}
-	T den = shift * shift + T(1e-28);
-#pragma unroll
-	for (int i = 0; i < NF; ++i) {
-		T nu = T(p.nu[i]) * shift;
-		T sq = M::sqrt_(nu);
-		T j  = M::div(n, sq);
-		T al = M::div(T(0.1) * n, (nu * nu) * sq);
-		dI[i]   = M::div(-(j * M::exp_(-tau[i])), den);
-		dtau[i] = -(al * shift);
-	}
-}
-
 } // namespace gray

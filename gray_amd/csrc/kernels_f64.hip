@@ -48,8 +48,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 4 ? 2
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = data[(size_t)i * n_rays + g];
+	const RtConst64<NF> rc = make_rtconst64<NF>(p);
 	bool retired;
-	int64_t st = interval_f64<NF>(p, sr, s, 0, &retired);
+	int64_t st = interval_f64<NF>(p, rc, sr, s, 0, &retired);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		data[(size_t)i * n_rays + g] = s[i];

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 	 * longest ray is the render's critical path: raise their issue
 	 * priority so the pool waves fill the gaps of their dependency chain. */
 	__builtin_amdgcn_s_setprio(3);
-	const RtConstFb<NF> rc = make_rtconst_fb<NF>(p);
+	const RtConst64<NF> rc = make_rtconst64<NF>(p);
 	Fallback f = fb[e];
 	size_t g = (size_t)f.g;
 	double s[ND];

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 	const int s = list[idx];
 	const SrvRequest &R = reqs[S.req[s]];
 	const StepRule sr = R.sr;
-	const RtConstFb<SRV_NF> rc = make_rtconst_fb<SRV_NF>(R.p);
+	const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
 	double x[SRV_NC];
 	int d = S.d[s], h = S.h[s];
 	int64_t st = S.steps[s];

@@ -1,0 +1,121 @@
+/*
+ * rt_disk.h -- the FP64 transfer RHS of the synthetic disk, evaluated from
+ * the Kerr-Schild quantities gr_rhs computes anyway, and the ~1 ulp FP64
+ * reciprocal and reciprocal square root it runs on.
+ *
+ * Used by the plain FP64 form (whose geodesic stays bit-exact: the transfer
+ * channels never feed back into the geodesic, and they are held to 1e-10
+ * relative, exp not being correctly rounded in any case) and by the FP64
+ * re-integration of the FP32 form.  Same mathematics as sim-org/rt.cl:185-198
+ * with the fluid of sim-org/flow.cl:36-58 replaced by the Keplerian disk,
+ * regrouped:
+ *   k_mu = eta u + f (l.u) l_mu,   g(v,v) = eta(v,v) + f (l.v)^2
+ * for the Keplerian v = (1, -y Omega, x Omega, 0), Omega = 1/(r^1.5 + a);
+ * s = -k.v / sqrt(-g(v,v)), n = r^-1.5 exp(-z^2/(2 (0.3 R)^2)) outside
+ * r_isco, and per channel
+ *   dI_i = -n nu_i^-1/2 s^-1/2 exp(-tau_i) / (s^2 + 1e-28)
+ *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2
+ * with nu_i^-1/2 and 0.1 nu_i^-5/2 formed once per ray.
+ */
+#pragma once
+
+#include "gray_device.h"
+
+namespace gray {
+
+/* ~1 ulp FP64 reciprocal, reciprocal square root, division and square root:
+ * the hardware estimates refined by two Newton steps. */
+struct MthFb {
+	static __device__ __forceinline__ double rcp(double b)
+	{
+		double r = __builtin_amdgcn_rcp(b);
+		r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+		return __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+	}
+	static __device__ __forceinline__ double rsq(double x)
+	{
+		double y = __builtin_amdgcn_rsq(x);
+		/* y (3 - x y^2) / 2, twice */
+		y = y * __builtin_fma(-0.5 * x * y, y, 1.5);
+		return y * __builtin_fma(-0.5 * x * y, y, 1.5);
+	}
+	static __device__ __forceinline__ double div(double a, double b) { return a * rcp(b); }
+	static __device__ __forceinline__ double sqrt_(double x)
+	{
+		return x > 0.0 ? x * rsq(x) : __builtin_sqrt(x);
+	}
+	static __device__ __forceinline__ double exp_(double x) { return exp(x); }
+};
+
+template <int NF> struct RtConst64 {
+	double c[NF > 0 ? NF : 1], d[NF > 0 ? NF : 1];
+};
+
+template <int NF, class P>
+__device__ __forceinline__ RtConst64<NF> make_rtconst64(const P &p)
+{
+	RtConst64<NF> r;
+#pragma unroll
+	for (int i = 0; i < (NF > 0 ? NF : 1); ++i) {
+		/* a serving slot carries NF channels whatever its request asked
+		 * for: the unused ones repeat channel 0 (their results are dropped) */
+		const double nu = NF > 0 ? (double)p.nu[i < p.n_freq ? i : 0] : 1.0;
+		r.c[i] = 1.0 / sqrt(nu);
+		r.d[i] = 0.1 / (nu * nu * sqrt(nu));
+	}
+	return r;
+}
+
+/* K must come from gr_rhs at the same (q, u); its r is the r of getrr, so
+ * the r_isco cut is the one of the step rule's geometry.  Fills dI[0..NF),
+ * dtau[0..NF). */
+template <int NF, class P>
+__device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &rc,
+                                              const V4<double> &q, const V4<double> &u,
+                                              const KsAux<double> &K, const double *tau,
+                                              double *dI, double *dtau)
+{
+	typedef MthFb M;
+	double n = 0.0, sh = 1.0;
+	if (p.fluid == GRAYHIP_FLUID_DISK) {
+		const double R2 = q.x * q.x + q.y * q.y;
+		if (K.r >= p.r_isco && R2 > 0.0) {
+			const double isr = M::rsq(K.r);
+			const double rs = (K.r * K.r) * isr;
+			const double Om = M::rcp(rs + p.a);
+			const double lv = 1.0 + Om * (q.x * K.ly - q.y * K.lx);
+			const double vv = (Om * Om * R2 - 1.0) + K.f * lv * lv;
+			if (vv < 0.0) {
+				const double ut = M::rsq(-vv);
+				const double flu = K.f * K.lu;
+				const double kx = u.x + flu * K.lx, ky = u.y + flu * K.ly;
+				const double kv = (flu - u.t) + Om * (q.x * ky - q.y * kx);
+				const double s_ = -kv * ut;
+				if (s_ > 0.0) {
+					sh = s_;
+					n = (K.ir * isr) * M::exp_(-(q.z * q.z) * M::rcp(0.18 * R2));
+				}
+			}
+		}
+	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
+		n = 1.0;
+	}
+	if (n == 0.0) {
+#pragma unroll
+		for (int i = 0; i < NF; ++i) {
+			dI[i] = 0.0;
+			dtau[i] = 0.0;
+		}
+		return;
+	}
+	const double rsh = M::rsq(sh);
+	const double A = n * rsh * M::rcp(sh * sh + 1e-28);
+	const double Bc = n * (rsh * rsh) * rsh;
+#pragma unroll
+	for (int i = 0; i < NF; ++i) {
+		dI[i] = -(A * rc.c[i]) * M::exp_(-tau[i]);
+		dtau[i] = -(Bc * rc.d[i]);
+	}
+}
+
+} // namespace gray

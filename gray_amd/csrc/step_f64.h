@@ -7,37 +7,42 @@
 #pragma once
 
 #include "gray_device.h"
+#include "rt_disk.h"
 
 namespace gray {
 
+/* The geodesic in the written order with correctly rounded division and
+ * square root (bit-exact); the transfer channels from rt_disk.h. */
 template <int NF, class P>
-__device__ __forceinline__ void rhs_f64(const P &p, const double *s, double *ds)
+__device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, const double *s,
+                                        double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
-	V4<double> a = gr_rhs(p.a, q, u);
+	KsAux<double> K;
+	V4<double> a = gr_rhs(p.a, q, u, NF > 0 ? &K : nullptr);
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if (NF > 0)
-		rt_rhs<double, NF>(p, q, u, s + 8 + NF, ds + 8, ds + 8 + NF);
+		rt_rhs_disk64<NF, P>(p, rc, q, u, K, s + 8 + NF, ds + 8, ds + 8 + NF);
 }
 
 /* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at
  * s + (dt/2) k1, s + (dt/2) k2, s + dt k3. */
 template <int NF, class P>
-__device__ __forceinline__ void rk4_f64(const P &p, double *s, double dt)
+__device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, double *s, double dt)
 {
 	constexpr int ND = 8 + 2 * NF;
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
 
-	rhs_f64<NF, P>(p, s, k1);
+	rhs_f64<NF, P>(p, rc, s, k1);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
-	rhs_f64<NF, P>(p, x, k23);
+	rhs_f64<NF, P>(p, rc, x, k23);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
-	rhs_f64<NF, P>(p, x, k);
+	rhs_f64<NF, P>(p, rc, x, k);
 	/* k1 + 2 (k2 + k3) is formed before the last stage (same value), so
 	 * k1 and k2 + k3 are not both live across it */
 #pragma unroll
@@ -45,7 +50,7 @@ __device__ __forceinline__ void rk4_f64(const P &p, double *s, double dt)
 		k1[i] = k1[i] + 2.0 * (k23[i] + k[i]);
 		x[i] = s[i] + dt * k[i];
 	}
-	rhs_f64<NF, P>(p, x, k);
+	rhs_f64<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = s[i] + (dt * (k1[i] + k[i])) / 6.0;
@@ -54,8 +59,9 @@ __device__ __forceinline__ void rk4_f64(const P &p, double *s, double dt)
 /* Sub-steps of one interval from unit h0 (sim-org/driver.cl:92-99).
  * Returns steps taken; *retired set when the step rule returned zero. */
 template <int NF, class P>
-__device__ __forceinline__ int64_t interval_f64(const P &p, const StepRule &sr,
-                                                double *s, int h0, bool *retired)
+__device__ __forceinline__ int64_t interval_f64(const P &p, const RtConst64<NF> &rc,
+                                                const StepRule &sr, double *s, int h0,
+                                                bool *retired)
 {
 	int64_t st = 0;
 	int dh;
@@ -71,7 +77,7 @@ __device__ __forceinline__ int64_t interval_f64(const P &p, const StepRule &sr,
 		}
 		if (dh > sr.n - h)
 			dh = sr.n - h;
-		rk4_f64<NF, P>(p, s, (double)dh * sr.dt / (double)sr.n);
+		rk4_f64<NF, P>(p, rc, s, (double)dh * sr.dt / (double)sr.n);
 		++st;
 	}
 	return st;
