@@ -48,7 +48,8 @@ struct SrvLists {
 	SrvCounters *ctr;
 };
 
-/* Per-lane parameters of the FP32 Kerr path (the fields getdt/rt_rhs use). */
+/* Per-lane parameters of the FP32 Kerr path (the fields getdt and the
+ * transfer RHS use). */
 struct LaneKerr {
 	float a, r_h, eps_stop, r_esc, r_isco;
 	int escape, fluid, n_freq;
