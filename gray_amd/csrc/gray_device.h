@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fpdiv.h"
 #include "grayhip_params.h"
 
 namespace gray {

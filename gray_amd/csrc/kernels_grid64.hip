@@ -174,7 +174,7 @@ __device__ __forceinline__ void grid_rk4_brick_f64(const GridView<S> &G, const R
 	grid_rhs_brick_f64<NF, S>(G, rt, brick, b, use_lds, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
-		s[i] = s[i] + (dt * ((k1[i] + 2.0 * k2[i]) + k[i])) / 6.0;
+		s[i] = s[i] + div6(dt * ((k1[i] + 2.0 * k2[i]) + k[i]));
 }
 
 template <int NF, class S>

@@ -53,7 +53,7 @@ __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, dou
 	rhs_f64<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
-		s[i] = s[i] + (dt * (k1[i] + k[i])) / 6.0;
+		s[i] = s[i] + div6(dt * (k1[i] + k[i]));
 }
 
 /* Sub-steps of one interval from unit h0 (sim-org/driver.cl:92-99).
