@@ -32,7 +32,10 @@ namespace gray {
 
 constexpr int GT_WAVES = 2;                 /* waves per block */
 
-/* sample_lev_f64 with the corners in the brick (same sums, same order). */
+/* sample_lev_f64 with the corners in the brick (same sums, same order).
+ * Unrolled, so the LDS reads of later corners issue while earlier ones are
+ * summed; the global-memory corner loop stays rolled (unrolled, its
+ * registers cost more than its overlap gains). */
 template <int NC, class S>
 __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, int base,
                                                  const Stencil<double> &st, const int o[3],
@@ -47,7 +50,7 @@ __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, in
 		for (int c = 0; c < NC; ++c)
 			out[c] = w * (double)p[c];
 	}
-#pragma unroll 1
+#pragma unroll
 	for (int k = 1; k < 8; ++k) {
 		const int pt = p0 + ((k & 1) ? ox : 0) + ((k & 2) ? oy : 0) + ((k & 4) ? oz : 0);
 		const S *p = brick + pt * 40 + base;
