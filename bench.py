@@ -59,11 +59,13 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
 # HBM bytes per launch of the dominant kernels on these workloads, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by what the same
-# counters read for a 256 MiB device copy (read x1.9998, write x1.0000):
+# counters read for a 256 MiB device copy (read x1.9997, write x1.0000):
 # profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
-# profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float>).
-TRAFFIC_POOL_PASS = (73.87 + 79.50) * 2**20
-TRAFFIC_GRID_TILES = (629.26 + 37.51) * 2**20
+# profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
+# k_grid_tiles_f64<0, double>).
+TRAFFIC_POOL_PASS = (78.39 + 80.38) * 2**20
+TRAFFIC_GRID_TILES = (632.64 + 37.54) * 2**20
+TRAFFIC_GRID_TILES64 = (4110.56 + 21.36) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
 # Algorithmic bytes per gridded ray-step: 4 RHS x 40 Christoffels x 8
 # corners x 2 time levels x bytes per stored real.
@@ -191,8 +193,8 @@ def grid_section(G, out):
             "roofline": {"bound": "hbm", "kernel": "k_grid_tiles_f32<0, float>" if dtype == 4 else "k_grid_tiles_f64<0, double>",
                          "achieved": gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / PEAK_HBM_GBPS,
-                         "traffic": TRAFFIC_GRID_TILES if dtype == 4 else None,
-                         "traffic_source": TRAFFIC_SOURCE if dtype == 4 else None,
+                         "traffic": TRAFFIC_GRID_TILES if dtype == 4 else TRAFFIC_GRID_TILES64,
+                         "traffic_source": TRAFFIC_SOURCE,
                          "bytes_per_ray_step": GRID_B[dtype]},
         }
     (d32, s32), (d64, s64) = finals[4], finals[8]
