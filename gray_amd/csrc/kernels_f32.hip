@@ -39,6 +39,9 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 	const unsigned lane = __lane_id();
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
+	/* the quantised step count is gdt n / dt; the FP32 form takes it as
+	 * gdt (n / dt), one multiply instead of a division per step */
+	const double n_over_dt = (double)sr.n / sr.dt;
 	const RtConst<NF> rc = make_rtconst<NF>(p);
 
 	float hi[ND], lo[ND];
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
 		float gdt = getdt(p, q, u, dts);
-		double xd = (double)gdt / sr.dt * (double)sr.n;
+		double xd = (double)gdt * n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		bool finish = (dh == 0);
 		if (!finish) {

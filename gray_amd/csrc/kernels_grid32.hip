@@ -276,12 +276,13 @@ void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__rest
 	const int64_t st0 = st;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
+	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
 	for (int d = 0; d < sr.n_dump; ++d) {
 		int dh;
 		bool retired = false;
 		for (int h = 0; h < sr.n; h += dh) {
 			float gdt = grid_getdt<float, S>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
-			double xd = (double)gdt / sr.dt * (double)sr.n;
+			double xd = (double)gdt * n_over_dt;
 			dh = (xd == xd) ? (int)xd : 0;
 			if (!dh) {
 				retired = true;

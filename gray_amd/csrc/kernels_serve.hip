@@ -148,12 +148,13 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 	const int64_t st0 = st;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
+	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
 	int state = 0; /* 0 running, 1 finished, 2 to FP64, 3 restart in FP64 */
 	for (int k = 0; k < K; ++k) {
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
 		float gdt = getdt(P, q, u, dts);
-		double xd = (double)gdt / sr.dt * (double)sr.n;
+		double xd = (double)gdt * n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		if (dh == 0) {
 			state = 1;
@@ -239,12 +240,13 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 			x[c] = S.st64[(size_t)c * S.S + s];
 	}
 	const int64_t st0 = st;
+	const double n_over_dt = (double)sr.n / sr.dt;
 	bool fin = false;
 	int dh;
 	for (int k = 0; k < K; ++k) {
 		V4<double> q = {x[0], x[1], x[2], x[3]};
 		V4<double> u = {x[4], x[5], x[6], x[7]};
-		double xd = getdt(R.p, q, u, sr.dts) / sr.dt * (double)sr.n;
+		double xd = getdt(R.p, q, u, sr.dts) * n_over_dt;
 		dh = (xd == xd) ? (int)xd : 0;
 		if (!dh) {
 			fin = true;
@@ -300,10 +302,11 @@ __global__ __launch_bounds__(256) void k_srv_grid32(const SrvRequest *__restrict
 	const int64_t st0 = st;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
+	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
 	bool fin = false;
 	for (int k = 0; k < K; ++k) {
 		float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
-		double xd = (double)gdt / sr.dt * (double)sr.n;
+		double xd = (double)gdt * n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		if (!dh) {
 			fin = true;

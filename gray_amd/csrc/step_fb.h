@@ -9,6 +9,7 @@
  *     reciprocal square root refined by two Newton steps (MthFb of
  *     rt_disk.h: about 1 ulp instead of correctly rounded);
  *   - the transfer RHS of rt_disk.h;
+ *   - the quantised step count as getdt (n / dt), one multiply;
  *   - the RK4 combination multiplied by 1/6.
  * The arithmetic is the same mathematics as sim-org/KS.cl:263-378,
  * rt.cl:185-198 and RK4.cl:37-47 in a different evaluation order.
@@ -69,10 +70,11 @@ __device__ __forceinline__ int64_t interval_fb(const P &p, const RtConst64<NF> &
 	int64_t st = 0;
 	int dh;
 	*retired = false;
+	const double n_over_dt = (double)sr.n / sr.dt;
 	for (int h = h0; h < sr.n; h += dh) {
 		V4<double> q = {s[0], s[1], s[2], s[3]};
 		V4<double> u = {s[4], s[5], s[6], s[7]};
-		double x = getdt(p, q, u, sr.dts) / sr.dt * (double)sr.n;
+		double x = getdt(p, q, u, sr.dts) * n_over_dt;
 		dh = (x == x) ? (int)x : 0;
 		if (!dh) {
 			*retired = true;
