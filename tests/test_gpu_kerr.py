@@ -3,8 +3,10 @@ against the FP64 oracle.
 
 Bar: the plain FP64 form must equal the oracle bit for bit on the geodesic
 state and step counts (it evaluates the same correctly rounded sequence);
-transfer channels use exp(), which is not correctly rounded on either side,
-so they are held to 1e-10 relative.  The FP32 form is held to image relative
+transfer channels, which never feed back into the geodesic, are evaluated in
+a regrouped form from the Kerr-Schild quantities of the geodesic RHS and use
+exp(), which is not correctly rounded on either side, so they are held to
+1e-10 relative.  The FP32 form is held to image relative
 L1 <= 1e-4 per channel against the oracle.
 """
 import numpy as np
