@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
                                                   Fallback *__restrict__ fb)
 {
 	const unsigned n0 = (unsigned)(rr.e0 - rr.b0), n_all = n0 + (unsigned)(rr.e1 - rr.b1);
-	constexpr int ND = 8 + 2 * NF;
+	constexpr int ND = nd32<NF>();
 	const unsigned lane = __lane_id();
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
@@ -64,12 +64,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 				unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
 				if (idx < n_all) {
 					g = idx < n0 ? (long)rr.b0 + idx : (long)rr.b1 + (idx - n0);
-#pragma unroll
-					for (int i = 0; i < ND; ++i) {
-						double v = data[(size_t)i * n_rays + g];
-						hi[i] = (float)v;
-						lo[i] = (float)(v - (double)hi[i]);
-					}
+					load_state<NF>(p, data, n_rays, g, hi, lo);
 					st = steps[g];
 					st0 = st;
 					d = 0;
@@ -94,7 +89,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			double hd = (double)dh * dt_over_n;
 			if (fabs(hd) < 1e-4) {
 				/* step collapse: FP64 re-integration from here */
-				store_state<NF>(data, n_rays, g, hi, lo);
+				store_state<NF>(p, data, n_rays, g, hi, lo);
 				steps[g] = st;
 				atomicAdd(&ctr->pool_steps, (unsigned long long)(st - st0));
 				unsigned slot = atomicAdd(&ctr->n_fb, 1u);
@@ -120,7 +115,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			}
 		}
 		if (finish) {
-			store_state<NF>(data, n_rays, g, hi, lo);
+			store_state<NF>(p, data, n_rays, g, hi, lo);
 			V4<double> qd = {(double)hi[0] + (double)lo[0], (double)hi[1] + (double)lo[1],
 			                 (double)hi[2] + (double)lo[2], (double)hi[3] + (double)lo[3]};
 			V4<double> ud = {(double)hi[4] + (double)lo[4], (double)hi[5] + (double)lo[5],

@@ -137,9 +137,12 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 	const LaneKerr P = lane_params(R.p);
 	const RtConst<SRV_NF> rc = make_rtconst<SRV_NF>(P);
 	const StepRule sr = R.sr;
-	float hi[SRV_NC], lo[SRV_NC];
+	/* the slot holds the FP32 form's state (geodesic, intensities, the
+	 * shared depth integral T in channel 0's depth place; step_f32.h) */
+	constexpr int NI = nd32<SRV_NF>();
+	float hi[NI], lo[NI];
 #pragma unroll
-	for (int c = 0; c < SRV_NC; ++c) {
+	for (int c = 0; c < NI; ++c) {
 		hi[c] = S.st[(size_t)c * S.S + s];
 		lo[c] = S.st[(size_t)(SRV_NC + c) * S.S + s];
 	}
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 	atomicAdd(&out.ctr->steps[0], (unsigned long long)(st - st0));
 	if (state == 0) {
 #pragma unroll
-		for (int c = 0; c < SRV_NC; ++c) {
+		for (int c = 0; c < NI; ++c) {
 			S.st[(size_t)c * S.S + s] = hi[c];
 			S.st[(size_t)(SRV_NC + c) * S.S + s] = lo[c];
 		}
@@ -196,9 +199,7 @@ __global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict
 		return;
 	}
 	double x[SRV_NC];
-#pragma unroll
-	for (int c = 0; c < SRV_NC; ++c)
-		x[c] = (double)hi[c] + (double)lo[c];
+	join_state<SRV_NF>(R.p, hi, lo, x);
 	if (state == 1) {
 		V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
 		write_out(R, S.ray[s], x, SRV_NF, getuu(R.p.a, qd, ud), st);

@@ -2,12 +2,23 @@
  * step_f32.h -- one ray in the FP32 form: FP32 state derivative and the
  * classical RK4 step on a compensated (hi + lo) state.  P is the parameter
  * block (the render's RayParams or per-lane serving parameters).
+ *
+ * The synthetic disk's absorption is a fixed power of frequency, alpha_i
+ * shift = 0.1 nu_i^-5/2 (n shift^-3/2), so every channel's optical depth is
+ * tau_i = d_i T with d_i = 0.1 nu_i^-5/2 and one shared integral
+ * T = -int n shift^-3/2 dlambda.  The FP32 form integrates T once instead of
+ * the n_freq depths: its state is 8 geodesic reals, the n_freq intensities
+ * and T (ND32 reals); load_state and store_state convert from and to the
+ * stored layout (8 + 2 n_freq reals, tau_i in place).
  */
 #pragma once
 
 #include "gray_device.h"
 
 namespace gray {
+
+/* Reals of the FP32 form's state: geodesic, intensities, T. */
+template <int NF> constexpr int nd32() { return 8 + NF + (NF > 0 ? 1 : 0); }
 
 /* Per-channel constants of the synthetic emissivity, nu_i^-1/2 and
  * 0.1 nu_i^-5/2, so that a transfer RHS needs one rsqrt for the redshift
@@ -34,11 +45,12 @@ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
  *   k_mu = eta u + f (l.u) l_mu,   g(v,v) = eta(v,v) + f (l.v)^2
  * for the Keplerian v = (1, -y Omega, x Omega, 0); then
  *   dI_i = -n nu_i^-1/2 s^-1/2 exp(-tau_i) / (s^2 + 1e-28)
- *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2       (s = -k.u_fluid). */
+ *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2       (s = -k.u_fluid),
+ * with tau_i = d_i T and dT = -n s^-3/2. */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, const V4<float> &q,
-                                           const V4<float> &u, const KsAux<float> &K,
-                                           const float *tau, float *dI, float *dtau)
+                                           const V4<float> &u, const KsAux<float> &K, float T,
+                                           float *dI, float &dT)
 {
 	typedef Mth<float> M;
 	float n = 0.f, sh = 1.f;
@@ -68,20 +80,17 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 	}
 	if (n == 0.f) {
 #pragma unroll
-		for (int i = 0; i < NF; ++i) {
+		for (int i = 0; i < NF; ++i)
 			dI[i] = 0.f;
-			dtau[i] = 0.f;
-		}
+		dT = 0.f;
 		return;
 	}
 	const float rsh = __builtin_amdgcn_rsqf(sh);
 	const float A = n * rsh * M::div(1.f, sh * sh + 1e-28f);
-	const float Bc = n * (rsh * rsh) * rsh;
+	dT = -(n * (rsh * rsh) * rsh);
 #pragma unroll
-	for (int i = 0; i < NF; ++i) {
-		dI[i] = -(A * rc.c[i]) * M::exp_(-tau[i]);
-		dtau[i] = -(Bc * rc.d[i]);
-	}
+	for (int i = 0; i < NF; ++i)
+		dI[i] = -(A * rc.c[i]) * M::exp_(-(rc.d[i] * T));
 }
 
 template <int NF, class P>
@@ -95,7 +104,7 @@ __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if (NF > 0)
-		rt_rhs_f32<NF, P>(p, rc, q, u, K, s + 8 + NF, ds + 8, ds + 8 + NF);
+		rt_rhs_f32<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
 /* Classical RK4 (sim-org/RK4.cl:37-47) with stages taken from hi and the
@@ -104,7 +113,7 @@ template <int NF, class P>
 __device__ __forceinline__ void rk4_f32c(const P &p, const RtConst<NF> &rc, float *hi, float *lo,
                                          float dt)
 {
-	constexpr int ND = 8 + 2 * NF;
+	constexpr int ND = nd32<NF>();
 	float k[ND], acc[ND], x[ND];
 	const float hdt = 0.5f * dt;
 
@@ -137,14 +146,67 @@ __device__ __forceinline__ void rk4_f32c(const P &p, const RtConst<NF> &rc, floa
 	}
 }
 
-template <int NF>
-__device__ __forceinline__ void store_state(double *__restrict__ data, size_t n_rays, size_t g,
-                                            const float *hi, const float *lo)
+/* d_i = 0.1 nu_i^-5/2 in FP64 (tau_i = d_i T); channels a serving slot
+ * carries beyond its request's repeat channel 0. */
+template <class P>
+__device__ __forceinline__ double tau_scale(const P &p, int i)
 {
-	constexpr int ND = 8 + 2 * NF;
+	const double nu = (double)p.nu[i < p.n_freq ? i : 0];
+	return 0.1 / (nu * nu * sqrt(nu));
+}
+
+/* Stored layout (x: 8 + 2 NF reals) -> FP32-form state (hi + lo, ND32
+ * reals): T from channel 0's depth. */
+template <int NF, class P>
+__device__ __forceinline__ void split_state(const P &p, const double *x, float *hi, float *lo)
+{
 #pragma unroll
-	for (int i = 0; i < ND; ++i)
-		data[(size_t)i * n_rays + g] = (double)hi[i] + (double)lo[i];
+	for (int i = 0; i < 8 + NF; ++i) {
+		hi[i] = (float)x[i];
+		lo[i] = (float)(x[i] - (double)hi[i]);
+	}
+	if (NF > 0) {
+		const double T = x[8 + NF] / tau_scale(p, 0);
+		hi[8 + NF] = (float)T;
+		lo[8 + NF] = (float)(T - (double)hi[8 + NF]);
+	}
+}
+
+/* FP32-form state -> stored layout (8 + 2 NF reals, tau_i = d_i T). */
+template <int NF, class P>
+__device__ __forceinline__ void join_state(const P &p, const float *hi, const float *lo, double *x)
+{
+#pragma unroll
+	for (int i = 0; i < 8 + NF; ++i)
+		x[i] = (double)hi[i] + (double)lo[i];
+	if (NF > 0) {
+		const double T = (double)hi[8 + NF] + (double)lo[8 + NF];
+#pragma unroll
+		for (int i = 0; i < NF; ++i)
+			x[8 + NF + i] = tau_scale(p, i) * T;
+	}
+}
+
+template <int NF, class P>
+__device__ __forceinline__ void load_state(const P &p, const double *__restrict__ data,
+                                           size_t n_rays, size_t g, float *hi, float *lo)
+{
+	double x[8 + 2 * NF];
+#pragma unroll
+	for (int i = 0; i < 8 + NF + (NF > 0 ? 1 : 0); ++i)
+		x[i] = data[(size_t)i * n_rays + g];
+	split_state<NF, P>(p, x, hi, lo);
+}
+
+template <int NF, class P>
+__device__ __forceinline__ void store_state(const P &p, double *__restrict__ data, size_t n_rays,
+                                            size_t g, const float *hi, const float *lo)
+{
+	double x[8 + 2 * NF];
+	join_state<NF, P>(p, hi, lo, x);
+#pragma unroll
+	for (int i = 0; i < 8 + 2 * NF; ++i)
+		data[(size_t)i * n_rays + g] = x[i];
 }
 
 } // namespace gray
