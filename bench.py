@@ -63,7 +63,7 @@ PEAK_HBM_GBPS = 8000.0
 # profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
 # profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
 # k_grid_tiles_f64<0, double>).
-TRAFFIC_POOL_PASS = (78.39 + 80.38) * 2**20
+TRAFFIC_POOL_PASS = (64.10 + 80.41) * 2**20
 TRAFFIC_GRID_TILES = (632.64 + 37.54) * 2**20
 TRAFFIC_GRID_TILES64 = (4110.56 + 21.36) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
