@@ -232,7 +232,7 @@ __device__ __forceinline__ void grid_rk4_brick(const GridView<S> &G, const RtThe
  * cells they read stay in L2.  Blocks reach the 8 XCDs round robin, which
  * gives every XCD an even share of the costly tiles near the hole. */
 template <int NF, class S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF == 0 ? 3 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF == 0 ? 2 : 1)))
 void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__restrict__ data,
                       int64_t *__restrict__ steps, size_t n_rays, int W, int H, int bricks,
                       unsigned long long *__restrict__ n_steps)
