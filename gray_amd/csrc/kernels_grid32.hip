@@ -23,21 +23,31 @@ namespace gray {
  * A wave holds an 8x8 tile of the image (a bundle of neighbouring rays)
  * and runs every dump interval of its rays in registers; a ray that ends
  * leaves the wave's active mask, and the wave's decisions below are
- * ballots over the rays still active.  Every wave keeps a brick of 4x4x4
- * grid points (3x3x3 cells) of the t1 level's Christoffels in LDS
- * (64 x 160 B); a lane whose 8 corners lie in the brick reads them from
- * LDS, the others from global memory.  When more than a quarter of the
- * wave's rays have left the brick, the wave reloads it around the cell of
- * its first such ray (one point per lane); if that still leaves half of
- * them outside (rays spread near the hole), reloading pauses for 8
- * evaluations.  The brick's origin lives in LDS with it.  LDS holds the
+ * ballots over the rays still active.  Every wave keeps a brick of 5x5x5
+ * grid points (4x4x4 cells) of the t1 level's Christoffels in LDS
+ * (125 x 160 B = 20 KB; four waves per block, two blocks per CU at the
+ * kernel's two waves per SIMD); a lane whose 8 corners lie in the brick
+ * reads them from LDS, the others from global memory.  When more than a
+ * quarter of the wave's rays have left the brick, the wave reloads it
+ * around the cell of its first such ray (about two points per lane); if
+ * that still leaves half of them outside (rays spread near the hole),
+ * reloading pauses for 8 evaluations.  The brick's origin lives in LDS with it.  LDS holds the
  * same FP32 values as global memory and the contraction order is the
  * global path's, so a ray's result does not depend on where its corners
  * were read (bricks switched off give the same bits).
  * ------------------------------------------------------------------ */
 
 constexpr int GP_WAVES = 4;                 /* waves per block             */
-constexpr int GP_PTS = 64;                  /* 4x4x4 brick points          */
+constexpr int GB = 5;                       /* brick edge in grid points   */
+constexpr int GP_PTS = GB * GB * GB;        /* brick points                */
+
+/* Cell c's 8 corners lie in the brick with origin point o. */
+__device__ __forceinline__ bool in_brick5(const int c[3], const int o[3])
+{
+	return (unsigned)(c[0] - o[0]) <= (unsigned)(GB - 2) &&
+	       (unsigned)(c[1] - o[1]) <= (unsigned)(GB - 2) &&
+	       (unsigned)(c[2] - o[2]) <= (unsigned)(GB - 2);
+}
 
 struct BrickHdr {
 	int o[3];                                /* brick origin (point index)  */
@@ -51,8 +61,8 @@ struct BrickHdr {
 __device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, const Stencil<float> &st,
                                              const int o[3], const float P[10], float out[4])
 {
-	const float4 *p0 = brick + ((st.c[0] - o[0]) + 4 * (st.c[1] - o[1]) + 16 * (st.c[2] - o[2])) * 10;
-	const int ox = st.sx ? 10 : 0, oy = st.sy ? 40 : 0, oz = st.sz ? 160 : 0;
+	const float4 *p0 = brick + ((st.c[0] - o[0]) + GB * (st.c[1] - o[1]) + GB * GB * (st.c[2] - o[2])) * 10;
+	const int ox = st.sx ? 10 : 0, oy = st.sy ? 10 * GB : 0, oz = st.sz ? 10 * GB * GB : 0;
 	const float gx[2] = {1.f - st.f[0], st.f[0]};
 	const float gy[2] = {1.f - st.f[1], st.f[1]};
 	const float gz[2] = {1.f - st.f[2], st.f[2]};
@@ -121,7 +131,7 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 		o[1] = __builtin_amdgcn_readfirstlane(hdr->o[1]);
 		o[2] = __builtin_amdgcn_readfirstlane(hdr->o[2]);
 		const int cool = __builtin_amdgcn_readfirstlane(hdr->cool);
-		inside = in_brick(st.c, o);
+		inside = in_brick5(st.c, o);
 		const unsigned long long act = __ballot(1);
 		const unsigned long long out = __ballot(!inside);
 		const unsigned lane = __lane_id();
@@ -135,7 +145,7 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 			const int first = __ffsll((long long)out) - 1;
 			for (int a = 0; a < 3; ++a) {
 				const int c = __builtin_amdgcn_readfirstlane(__shfl(st.c[a], first));
-				const int hi_o = G.n[a] - 4;
+				const int hi_o = G.n[a] - GB;
 				o[a] = c - 1 < 0 ? 0 : (c - 1 > hi_o ? hi_o : c - 1);
 			}
 			wave_sync();
@@ -143,12 +153,12 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 			const int r = __popcll(act & ((1ull << lane) - 1ull));
 			const uint32_t nx = (uint32_t)G.n[0], nxy = nx * (uint32_t)G.n[1];
 			for (int p = r; p < GP_PTS; p += n_act) {
-				const uint32_t pt = (uint32_t)(o[2] + (p >> 4)) * nxy +
-				                    (uint32_t)(o[1] + ((p >> 2) & 3)) * nx +
-				                    (uint32_t)(o[0] + (p & 3));
+				const uint32_t pt = (uint32_t)(o[2] + p / (GB * GB)) * nxy +
+				                    (uint32_t)(o[1] + (p / GB) % GB) * nx +
+				                    (uint32_t)(o[0] + p % GB);
 				brick_store<S>(brick, p, G.gam[0] + (size_t)pt * 40);
 			}
-			inside = in_brick(st.c, o);
+			inside = in_brick5(st.c, o);
 			const int still = __popcll(__ballot(!inside));
 			if ((int)lane == leader) {
 				hdr->o[0] = o[0];
@@ -260,7 +270,7 @@ void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__rest
 		return;
 	const size_t g = (size_t)y * (size_t)W + x;
 
-	const bool use_lds = bricks && G.n[0] >= 4 && G.n[1] >= 4 && G.n[2] >= 4;
+	const bool use_lds = bricks && G.n[0] >= GB && G.n[1] >= GB && G.n[2] >= GB;
 	const RtThermalF rt = rt_thermal_f32(rt64);
 	const double inv_scale = NF > 0 ? 1.0 / rt64.scale : 1.0;
 	float hi[ND], lo[ND];
