@@ -64,8 +64,8 @@ PEAK_HBM_GBPS = 8000.0
 # profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
 # k_grid_tiles_f64<0, double>).
 TRAFFIC_POOL_PASS = (64.10 + 80.41) * 2**20
-TRAFFIC_GRID_TILES = (377.58 + 20.25) * 2**20
-TRAFFIC_GRID_TILES64 = (4071.21 + 21.31) * 2**20
+TRAFFIC_GRID_TILES = (525.41 + 21.40) * 2**20
+TRAFFIC_GRID_TILES64 = (4012.52 + 21.32) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
 # Algorithmic bytes per gridded ray-step: 4 RHS x 40 Christoffels x 8
 # corners x 2 time levels x bytes per stored real.
