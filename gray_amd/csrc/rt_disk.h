@@ -66,17 +66,18 @@ __device__ __forceinline__ RtConst64<NF> make_rtconst64(const P &p)
 	return r;
 }
 
-/* K must come from gr_rhs at the same (q, u); its r is the r of getrr, so
- * the r_isco cut is the one of the step rule's geometry.  Fills dI[0..NF),
- * dtau[0..NF). */
-template <int NF, class P>
-__device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &rc,
-                                              const V4<double> &q, const V4<double> &u,
-                                              const KsAux<double> &K, const double *tau,
-                                              double *dI, double *dtau)
+/* Density n and redshift factor sh of the disk at (q, u); n = 0 where
+ * nothing is emitted.  K must come from gr_rhs at the same (q, u); its r is
+ * the r of getrr, so the r_isco cut is the one of the step rule's
+ * geometry. */
+template <class P>
+__device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
+                                                const V4<double> &u, const KsAux<double> &K,
+                                                double &n, double &sh)
 {
 	typedef MthFb M;
-	double n = 0.0, sh = 1.0;
+	n = 0.0;
+	sh = 1.0;
 	if (p.fluid == GRAYHIP_FLUID_DISK) {
 		const double R2 = q.x * q.x + q.y * q.y;
 		if (K.r >= p.r_isco && R2 > 0.0) {
@@ -100,6 +101,18 @@ __device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &r
 	} else if (p.fluid == GRAYHIP_FLUID_UNIFORM) {
 		n = 1.0;
 	}
+}
+
+/* Fills dI[0..NF), dtau[0..NF) from the channels' depths tau. */
+template <int NF, class P>
+__device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &rc,
+                                              const V4<double> &q, const V4<double> &u,
+                                              const KsAux<double> &K, const double *tau,
+                                              double *dI, double *dtau)
+{
+	typedef MthFb M;
+	double n, sh;
+	disk_emission64(p, q, u, K, n, sh);
 	if (n == 0.0) {
 #pragma unroll
 		for (int i = 0; i < NF; ++i) {
@@ -116,6 +129,33 @@ __device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &r
 		dI[i] = -(A * rc.c[i]) * M::exp_(-tau[i]);
 		dtau[i] = -(Bc * rc.d[i]);
 	}
+}
+
+/* The same with the channels' depths as one shared integral: the disk's
+ * absorption is a fixed power of frequency, so tau_i = d_i T with
+ * d_i = 0.1 nu_i^-5/2 (rc.d) and dT = -n sh^-3/2.  Fills dI[0..NF), dT. */
+template <int NF, class P>
+__device__ __forceinline__ void rt_rhs_disk64_shared(const P &p, const RtConst64<NF> &rc,
+                                                     const V4<double> &q, const V4<double> &u,
+                                                     const KsAux<double> &K, double T,
+                                                     double *dI, double &dT)
+{
+	typedef MthFb M;
+	double n, sh;
+	disk_emission64(p, q, u, K, n, sh);
+	if (n == 0.0) {
+#pragma unroll
+		for (int i = 0; i < NF; ++i)
+			dI[i] = 0.0;
+		dT = 0.0;
+		return;
+	}
+	const double rsh = M::rsq(sh);
+	const double A = n * rsh * M::rcp(sh * sh + 1e-28);
+	dT = -(n * (rsh * rsh) * rsh);
+#pragma unroll
+	for (int i = 0; i < NF; ++i)
+		dI[i] = -(A * rc.c[i]) * M::exp_(-(rc.d[i] * T));
 }
 
 } // namespace gray

@@ -11,6 +11,10 @@
 
 namespace gray {
 
+/* Reals the plain FP64 form integrates: geodesic, intensities and the
+ * channels' shared depth integral T (tau_i = d_i T, rt_disk.h). */
+template <int NF> constexpr int nd64() { return 8 + NF + (NF > 0 ? 1 : 0); }
+
 /* The geodesic in the written order with correctly rounded division and
  * square root (bit-exact); the transfer channels from rt_disk.h. */
 template <int NF, class P>
@@ -24,7 +28,7 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if (NF > 0)
-		rt_rhs_disk64<NF, P>(p, rc, q, u, K, s + 8 + NF, ds + 8, ds + 8 + NF);
+		rt_rhs_disk64_shared<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
 /* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at
@@ -32,7 +36,7 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 template <int NF, class P>
 __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, double *s, double dt)
 {
-	constexpr int ND = 8 + 2 * NF;
+	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
 
