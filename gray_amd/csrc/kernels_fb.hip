@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
                                                       const Fallback *__restrict__ fb,
                                                       Counters *__restrict__ ctr)
 {
-	constexpr int ND = 8 + 2 * NF;
+	constexpr int ND = nd64<NF>();
 	unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= ctr->n_fb)
 		return;
@@ -36,9 +36,12 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 	size_t g = (size_t)f.g;
 	double s[ND];
 	const int d0 = f.dump, h0 = f.h;
+	/* stored: 8 + 2 NF reals with tau_i in place; integrated: T = tau_0 / d_0 */
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = data[(size_t)i * n_rays + g];
+	if (NF > 0)
+		s[8 + NF] = s[8 + NF] / rc.d[0];
 	const int64_t st0 = steps[g];
 	int64_t st = st0;
 	for (int d = d0; d < sr.n_dump; ++d) {
@@ -48,8 +51,11 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 			break;
 	}
 #pragma unroll
-	for (int i = 0; i < ND; ++i)
+	for (int i = 0; i < 8 + NF; ++i)
 		data[(size_t)i * n_rays + g] = s[i];
+#pragma unroll
+	for (int i = 0; i < NF; ++i)
+		data[(size_t)(8 + NF + i) * n_rays + g] = rc.d[i] * s[8 + NF];
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
 	info[g] = getuu(p.a, q, u);

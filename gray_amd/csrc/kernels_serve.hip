@@ -240,6 +240,9 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 		for (int c = 0; c < SRV_NC; ++c)
 			x[c] = S.st64[(size_t)c * S.S + s];
 	}
+	/* x holds the stored layout (tau_i in place); the step integrates the
+	 * channels' shared depth T = tau_0 / d_0 in tau_0's place */
+	x[8 + SRV_NF] = x[8 + SRV_NF] / rc.d[0];
 	const int64_t st0 = st;
 	const double n_over_dt = (double)sr.n / sr.dt;
 	bool fin = false;
@@ -267,6 +270,12 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 		}
 	}
 	atomicAdd(&out.ctr->steps[1], (unsigned long long)(st - st0));
+	{
+		const double T = x[8 + SRV_NF];
+#pragma unroll
+		for (int i = 0; i < SRV_NF; ++i)
+			x[8 + SRV_NF + i] = rc.d[i] * T;
+	}
 	if (fin) {
 		V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
 		write_out(R, S.ray[s], x, SRV_NF, getuu(R.p.a, qd, ud), st);

@@ -8,7 +8,8 @@
  *   - divisions and square roots from the hardware FP64 reciprocal and
  *     reciprocal square root refined by two Newton steps (MthFb of
  *     rt_disk.h: about 1 ulp instead of correctly rounded);
- *   - the transfer RHS of rt_disk.h;
+ *   - the transfer RHS of rt_disk.h, the channels' depths as one shared
+ *     integral T (the state is nd64 reals, as the plain form's);
  *   - the quantised step count as getdt (n / dt), one multiply;
  *   - the RK4 combination multiplied by 1/6.
  * The arithmetic is the same mathematics as sim-org/KS.cl:263-378,
@@ -18,6 +19,7 @@
 
 #include "gray_device.h"
 #include "rt_disk.h"
+#include "step_f64.h"
 
 namespace gray {
 
@@ -32,14 +34,14 @@ __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc, cons
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if (NF > 0)
-		rt_rhs_disk64<NF, P>(p, rc, q, u, K, s + 8 + NF, ds + 8, ds + 8 + NF);
+		rt_rhs_disk64_shared<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
 /* sim-org/RK4.cl:37-47 over rhs_fb. */
 template <int NF, class P>
 __device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, double *s, double dt)
 {
-	constexpr int ND = 8 + 2 * NF;
+	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
 	rhs_fb<NF, P>(p, rc, s, k1);
