@@ -50,16 +50,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 4 ? 2
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = data[(size_t)i * n_rays + g];
-	if (NF > 0)
+	if constexpr (NF > 0)
 		s[8 + NF] = s[8 + NF] / rc.d[0];
 	bool retired;
 	int64_t st = interval_f64<NF>(p, rc, sr, s, 0, &retired);
 #pragma unroll
 	for (int i = 0; i < 8 + NF; ++i)
 		data[(size_t)i * n_rays + g] = s[i];
+	if constexpr (NF > 0) {
 #pragma unroll
-	for (int i = 0; i < NF; ++i)
-		data[(size_t)(8 + NF + i) * n_rays + g] = rc.d[i] * s[8 + NF];
+		for (int i = 0; i < NF; ++i)
+			data[(size_t)(8 + NF + i) * n_rays + g] = rc.d[i] * s[8 + NF];
+	}
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
 	info[g] = getuu(p.a, q, u);

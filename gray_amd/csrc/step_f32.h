@@ -103,7 +103,7 @@ __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const
 	V4<float> a = gr_rhs((float)p.a, q, u, NF > 0 ? &K : nullptr);
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
-	if (NF > 0)
+	if constexpr (NF > 0)
 		rt_rhs_f32<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
@@ -165,7 +165,7 @@ __device__ __forceinline__ void split_state(const P &p, const double *x, float *
 		hi[i] = (float)x[i];
 		lo[i] = (float)(x[i] - (double)hi[i]);
 	}
-	if (NF > 0) {
+	if constexpr (NF > 0) {
 		const double T = x[8 + NF] / tau_scale(p, 0);
 		hi[8 + NF] = (float)T;
 		lo[8 + NF] = (float)(T - (double)hi[8 + NF]);
@@ -179,7 +179,7 @@ __device__ __forceinline__ void join_state(const P &p, const float *hi, const fl
 #pragma unroll
 	for (int i = 0; i < 8 + NF; ++i)
 		x[i] = (double)hi[i] + (double)lo[i];
-	if (NF > 0) {
+	if constexpr (NF > 0) {
 		const double T = (double)hi[8 + NF] + (double)lo[8 + NF];
 #pragma unroll
 		for (int i = 0; i < NF; ++i)

@@ -27,7 +27,7 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 	V4<double> a = gr_rhs(p.a, q, u, NF > 0 ? &K : nullptr);
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
-	if (NF > 0)
+	if constexpr (NF > 0)
 		rt_rhs_disk64_shared<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
