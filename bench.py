@@ -16,9 +16,10 @@ workload and the FP32 image is compared with it (relative L1 per channel).
 Further sections (each can be skipped): the numerical spacetime (512^2 rays
 through a 128^3 boosted Kerr-Schild grid with two time levels, FP32 form on
 an FP32 grid and FP64 form on an FP64 grid, HBM roofline), a sample of the
-serving stream (Poisson arrivals of uneven requests through one pool, p50/p99
-latency), the CPU baseline, and (with --precision-form) the a=0.998, i=85,
-2048^2 precision comparison.
+serving stream (a stratified sample of uneven requests through one pool:
+p50/p99 latency, saturated throughput, and the same requests rendered alone
+one after another in the plain FP64 form and in the FP32 form), the a=0.998,
+i=85, 2048^2 precision comparison, and the CPU baseline.
 """
 import argparse
 import json
@@ -131,6 +132,9 @@ def headline(G, args, out):
         "traffic_source": TRAFFIC_SOURCE,
         "flop_per_ray_step": FLOP_PER_STEP,
         "kernel_ms": 1e3 * k_s,
+        "kernel_ms_is": "span from the first pool pass's start to the last pass's end "
+                        "(the passes overlap on two streams); per-dispatch durations are in "
+                        "the rocprofv3 kernel trace under profiles/",
     }
     return img32, info32, value
 
@@ -206,70 +210,174 @@ def grid_section(G, out):
     out["grid"] = res
 
 
+# Stratified sample of the seeded 2000-request stream (seed 0x47524159):
+# the first requests of each resolution in stream order, in proportion to
+# the stream's weights, so every size (1024^2 included) and both resident
+# grids are served.
+SERVE_STRATA = ((64, 10), (128, 10), (256, 6), (512, 4), (1024, 2))
+
+
+def serve_sample(SV):
+    o = SV.defaults()
+    want = dict(SERVE_STRATA)
+    got = {s: 0 for s in want}
+    picked = []
+    for it in SV.stream(o):
+        if got[it.side] < want[it.side]:
+            picked.append(it)
+            got[it.side] += 1
+    return o, picked
+
+
+def serve_grids(G, o):
+    """The two resident spacetimes of the stream (boosted Kerr-Schild a=0.7,
+    v=0.3 along x on uniform [-40,40]^3, levels 0 and 1), as FP32 and FP64
+    copies for the single-render forms."""
+    from gray_amd import spacetime as S
+    b = S.bks(1.0, 0.7, 0.3, (1, 0, 0))
+    out = {}
+    for k in range(2):
+        n = o.grid_n[k]
+        xs = S.axis(-40.0, 40.0, n)
+        lv = [S.bks_sample(b, float(l), xs, xs, xs, rho=True) for l in range(2)]
+        for dtype in (4, 8):
+            tp = np.float32 if dtype == 4 else np.float64
+            st = G.Spacetime((n,) * 3, (-40.0,) * 3, (40.0,) * 3, dtype=dtype)
+            for l, (g, m, rho) in enumerate(lv):
+                st.upload(l, float(l), g.astype(tp), m.astype(tp))
+                st.upload_rho(l, rho.astype(tp))
+            st.set_frozen(False)
+            out[(k, dtype)] = st
+        del lv
+    return out
+
+
+def render_item(G, it, nf, precision, grids):
+    """One request of the stream rendered alone (the request loop's
+    parameters, gray_serve.c fill_request); returns (data, ray-steps, wall s)."""
+    nus = NUS[:nf]
+    if it.grid >= 0:
+        st = grids[(it.grid, 4 if precision == G.FP32 else 8)]
+        r = G.Render(it.side, it.side, precision=precision, spacetime=st, t_init=1.0, nu=nus)
+        cam, sched = (32.0, 32.0, 35.0), (DT, N_SUB, 4)
+    else:
+        r = G.Render(it.side, it.side, it.a_spin, precision=precision, nu=nus)
+        cam, sched = (IMG, IMG, R_OBS), (DT, N_SUB, N_DUMP)
+    t0 = time.perf_counter()
+    r.icond(*cam, it.i_obs, 0.0)
+    r.render(*sched)
+    wall = time.perf_counter() - t0
+    d, _ = r.read_f64()
+    steps = int(r.steps().sum())
+    r.close()
+    return d, steps, wall
+
+
 def serving_section(G, args, capacity, cap_name, out):
     from gray_amd import serve as SV
-    # Mean work of the sample's requests: analytic Kerr at ~6.4e4 steps/ray,
-    # gridded at ~1e4 steps/ray; arrivals come at 1.2x the rate the given
-    # capacity (ray-steps/s) sustains.
-    o = SV.defaults(n_requests=args.serve_n, max_side=args.serve_max_side)
-    items = SV.stream(o)
+    o, items = serve_sample(SV)
+    nfs = [SV.served_channels(it) for it in items]
+    sizes = {s: sum(1 for it in items if it.side == s) for s, _ in SERVE_STRATA}
+    n_grid = sum(1 for it in items if it.grid >= 0)
+    # mean work per request (analytic Kerr ~6.4e4 steps/ray, gridded ~1e4):
+    # Poisson arrivals at 1.2x the rate the given capacity sustains
     est_steps = np.mean([i.side * i.side * (1e4 if i.grid >= 0 else 6.4e4) for i in items])
-    o.rate = 1.2 * capacity / est_steps
-    rep, lat = SV.run(o)
-    res = {
-        "sample": f"first {args.serve_n} requests of the seeded stream (seed 0x47524159, "
-                  f"image side capped at {args.serve_max_side}), Poisson rate "
-                  f"{o.rate:.3f}/s = 1.2x the measured {cap_name} capacity",
+    rate = 1.2 * capacity / est_steps
+    rng = np.random.default_rng(0x47524159)
+    t = np.cumsum(rng.exponential(1.0 / rate, len(items)))
+    for it, ta in zip(items, t):
+        it.arrival_s = float(ta - t[0])
+    rep, lat, res = SV.run_items(o, items, keep_results=True)
+    res_ = {
+        "sample": f"{len(items)} requests of the seeded stream (seed 0x47524159): the first of "
+                  f"each side in stream order, {sizes} ({n_grid} gridded on the 96^3 and 128^3 "
+                  f"grids); Poisson arrivals at {rate:.3f}/s = 1.2x the measured {cap_name} "
+                  f"capacity",
+        "grid_channels": "carried" if SV.GRID_CHANNELS else
+                         "none (gridded requests are served without their drawn channels)",
         "p50_ms": rep["p50_ms"], "p99_ms": rep["p99_ms"], "max_ms": rep["max_ms"],
         "wall_s": rep["wall_s"], "rays": rep["rays"], "ray_steps": rep["ray_steps"],
         "epochs": rep["epochs"], "value": rep["ray_steps_per_s"], "unit": "ray-steps/s",
     }
     # the same requests arriving together: the pool's saturated throughput
-    o.rate = 1e9
-    sat, _ = SV.run(o)
-    res["saturated"] = {"value": sat["ray_steps_per_s"], "unit": "ray-steps/s",
-                        "wall_s": sat["wall_s"], "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"],
-                        "epochs": sat["epochs"]}
-    # plain FP64 form, run serially on the first requests of the same stream
-    t_fp64, st_fp64 = 0.0, 0
-    for it in items[: args.serve_fp64_n]:
-        if it.grid >= 0:
-            continue
-        nus = NUS[: it.n_freq]
-        r = G.Render(it.side, it.side, it.a_spin, precision=G.FP64, nu=nus)
-        t0 = time.perf_counter()
-        r.icond(IMG, IMG, R_OBS, it.i_obs, 0.0)
-        r.render(DT, N_SUB, N_DUMP)
-        t_fp64 += time.perf_counter() - t0
-        st_fp64 += r.stats()[0]
-        r.close()
-    if st_fp64:
-        res["fp64_serial"] = {"value": st_fp64 / t_fp64, "unit": "ray-steps/s",
-                              "requests": f"analytic-Kerr requests among the first "
-                                          f"{args.serve_fp64_n}"}
-        res["speedup_saturated_vs_fp64_serial"] = sat["ray_steps_per_s"] / (st_fp64 / t_fp64)
-    out["serving"] = res
+    for it in items:
+        it.arrival_s = 0.0
+    sat, _, _ = SV.run_items(o, items)
+    served_steps = sum(sat["ray_steps"])
+    res_["saturated"] = {"value": sat["ray_steps_per_s"], "unit": "ray-steps/s",
+                         "wall_s": sat["wall_s"], "p50_ms": sat["p50_ms"],
+                         "p99_ms": sat["p99_ms"], "epochs": sat["epochs"],
+                         "ray_steps": served_steps}
+    # the same requests, one after another, rendered alone: the plain FP64
+    # form (the denominator of the serving target) and the single-render
+    # FP32 form (render pool / grid tile kernel)
+    grids = serve_grids(G, o)
+    try:
+        serial = {}
+        l1_max, l1_items = 0.0, 0
+        for prec, name in ((G.FP64, "fp64_serial"), (G.FP32, "fp32_serial")):
+            t_all, st_all = 0.0, 0
+            for k, it in enumerate(items):
+                d, steps, wall = render_item(G, it, nfs[k], prec, grids)
+                t_all += wall
+                st_all += steps
+                if prec == G.FP64 and nfs[k] > 0:
+                    I0, I1 = d[:, 8:8 + nfs[k]], res[k][:, 8:8 + nfs[k]]
+                    den = np.abs(I0).sum(0)
+                    ok = den > 0
+                    if ok.any():
+                        l1_max = max(l1_max, float((np.abs(I1 - I0).sum(0)[ok] / den[ok]).max()))
+                        l1_items += 1
+            serial[name] = {"value": st_all / t_all, "unit": "ray-steps/s", "wall_s": t_all,
+                            "ray_steps": st_all}
+    finally:
+        for st in grids.values():
+            st.close()
+    res_.update(serial)
+    res_["fp64_serial"]["requests"] = "the same sample, each rendered alone in the plain FP64 form"
+    res_["fp32_serial"]["requests"] = "the same sample, each rendered alone in the FP32 form"
+    res_["speedup_saturated_vs_fp64_serial"] = sat["ray_steps_per_s"] / serial["fp64_serial"]["value"]
+    res_["saturated_vs_fp32_serial"] = sat["ray_steps_per_s"] / serial["fp32_serial"]["value"]
+    res_["image_rel_l1_vs_fp64_max"] = l1_max
+    res_["image_rel_l1_requests"] = l1_items
+    out["serving"] = res_
     out["latency_ms"] = {"p50": rep["p50_ms"], "p99": rep["p99_ms"],
                          "source": "serving sample"}
 
 
 def precision_section(G, out):
+    """The precision workload (a=0.998, i=85, 2048^2 rays, 4 channels): FP32
+    form against the plain FP64 form on the whole image."""
     ra, w32, _ = render(G, G.FP32, w=2048, a=0.998, inc=85.0)
     d32, i32 = ra.read_f64()
     st32, nfb = ra.stats()
+    ps, fbs = ra.pass_steps()
+    kn, _ = ra.kernel_time()
     ra.close()
     rb, w64, _ = render(G, G.FP64, w=2048, a=0.998, inc=85.0)
     d64, i64 = rb.read_f64()
     st64, _ = rb.stats()
+    kn64, _ = rb.kernel_time()
     rb.close()
     esc = np.sqrt(np.sum(d64[:, 1:4] ** 2, axis=1)) > 999.0
+    a32 = ps * FLOP_PER_STEP / (kn * 1e-9) / 1e12
+    a64 = st64 * FLOP_PER_STEP / (kn64 * 1e-9) / 1e12
     out["precision_form"] = {
-        "workload": "a=0.998, i=85, 2048x2048 rays, disk transfer 4 channels",
+        "workload": "a=0.998, i=85, r_obs=1000, 2048x2048 rays, 64Mx64M plane, disk transfer "
+                    "4 channels, dt=-32 x 128 dumps, n_sub=1024",
         "fp32_value": st32 / w32, "fp64_value": st64 / w64, "speedup": (st32 / w32) / (st64 / w64),
+        "fp32_render_ms": 1e3 * w32, "fp64_render_ms": 1e3 * w64,
+        "ray_steps": {"fp32_form": st32, "fp64_form": st64, "fp32_pool": ps, "fp64_fallback": fbs},
         "image_rel_l1": l1(d32[:, 8:12], d64[:, 8:12]),
         "peak_pixel_err": float(np.abs(d32[:, 8:12] - d64[:, 8:12]).max() / np.abs(d64[:, 8:12]).max()),
         "null_drift_fp32": float(np.abs(i32[esc]).max()), "null_drift_fp64": float(np.abs(i64[esc]).max()),
+        "escaping_rays": int(esc.sum()),
         "fallback_rays": nfb,
+        "roofline_fp32_pool": {"kernel": "k_pool_f32<4>", "achieved": a32, "peak": PEAK_FP32_TFLOPS,
+                               "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS,
+                               "kernel_ms": 1e-6 * kn, "kernel_ms_is": "span of the pool passes"},
+        "roofline_fp64": {"kernel": "k_evolve_f64<4>", "achieved": a64, "peak": PEAK_FP64_TFLOPS,
+                          "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS},
     }
 
 
@@ -282,6 +390,22 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the host CPUs this process may use,
+    capped by OMP_NUM_THREADS when set (the GPU host's share of CPUs per
+    GPU); also returns the host's logical CPU count and the allowed count."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = nproc
+    n = allowed
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n, nproc, allowed
 
 
 def cpu_baseline(stride, n_threads):
@@ -307,11 +431,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-grid", action="store_true", help="skip the numerical-spacetime section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sample")
-    ap.add_argument("--precision-form", action="store_true",
-                    help="also run the a=0.998, i=85, 2048^2 precision comparison (~1 min)")
-    ap.add_argument("--serve-n", type=int, default=48)
-    ap.add_argument("--serve-max-side", type=int, default=512)
-    ap.add_argument("--serve-fp64-n", type=int, default=12)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the a=0.998, i=85, 2048^2 precision comparison (~40 s)")
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the reference "
@@ -377,21 +498,25 @@ def main():
             log(f"serving: p50 {out['serving']['p50_ms']:.0f} ms, "
                 f"p99 {out['serving']['p99_ms']:.0f} ms")
         optional("serving", serve)
-    if args.precision_form:
+    if not args.no_precision:
         def prec():
             precision_section(G, out)
             log("precision form done")
         optional("precision_form", prec)
     if not args.no_cpu:
         def cpu():
-            n_thr = min(os.cpu_count() or 1, 16)
+            n_thr, nproc, allowed = cpu_threads()
             rate, steps, dt, n = cpu_baseline(args.cpu_stride, n_thr)
             out["cpu_baseline"] = {
                 "value": rate, "unit": "ray-steps/s", "cores": n_thr, "kind": "port",
+                "host_nproc": nproc, "host_cpus_allowed": allowed,
+                "per_thread": rate / n_thr,
                 "sample": f"FP64 oracle (CPU restatement of the reference FP64 path, C, "
                           f"-O3 -ffp-contract=off, OpenMP over rays) on every {args.cpu_stride}th "
                           f"row and column ({n} rays, {steps} ray-steps, {dt:.1f} s) on "
-                          f"{n_thr} threads of {cpu_model()}",
+                          f"{n_thr} threads of {cpu_model()} (host: {nproc} logical CPUs, "
+                          f"{allowed} allowed to this process; OMP_NUM_THREADS="
+                          f"{os.environ.get('OMP_NUM_THREADS', 'unset')})",
             }
         optional("cpu_baseline", cpu)
     print(json.dumps(out), flush=True)

@@ -577,6 +577,22 @@ extern "C" int grayhip_read_bl(grayhip_ctx *c, double r_match, double *out)
 	return GRAYHIP_OK;
 }
 
+extern "C" int grayhip_read_covariant(grayhip_ctx *c, double *out)
+{
+	if (!c || !out)
+		return fail(c, GRAYHIP_EINVAL, "null argument");
+	HCK(c, hipSetDevice(c->cfg.device));
+	if (c->st)
+		HCK(c, launch_grid_covariant(grid_view(c), c->d_data, (double *)c->d_tmp, c->n_rays,
+		                             c->stream));
+	else
+		HCK(c, launch_covariant(c->p.a, c->d_data, (double *)c->d_tmp, c->n_rays, c->stream));
+	HCK(c, hipMemcpyAsync(out, c->d_tmp, c->n_rays * 8 * sizeof(double), hipMemcpyDeviceToHost,
+	                      c->stream));
+	HCK(c, hipStreamSynchronize(c->stream));
+	return GRAYHIP_OK;
+}
+
 extern "C" int grayhip_read_f64(grayhip_ctx *c, double *data, double *info)
 {
 	if (!c)
@@ -593,11 +609,52 @@ extern "C" int grayhip_read_f64(grayhip_ctx *c, double *data, double *info)
 	return GRAYHIP_OK;
 }
 
+/* The disk and uniform fluids absorb as a fixed power of frequency
+ * (alpha_i shift = 0.1 nu_i^-5/2 n shift^-3/2), so every channel's depth is
+ * tau_i = d_i T, d_i = 0.1 nu_i^-5/2, with one shared integral T; both forms
+ * integrate T alone, taken from tau_0 (rt_disk.h, step_f32.h).  A restart
+ * must therefore hold depths in that proportion: tau_i d_0 = tau_0 d_i to a
+ * few rounding errors of the dump's real size. */
+static int check_depths(grayhip_ctx *c, const void *data)
+{
+	const int nf = c->p.n_freq;
+	if (c->st || nf < 2 ||
+	    (c->p.fluid != GRAYHIP_FLUID_DISK && c->p.fluid != GRAYHIP_FLUID_UNIFORM))
+		return GRAYHIP_OK;
+	double d[GRAYHIP_MAX_FREQ];
+	for (int i = 0; i < nf; ++i) {
+		const double nu = c->p.nu[i];
+		d[i] = 0.1 / (nu * nu * sqrt(nu));
+	}
+	const int sz = c->cfg.precision, nd = c->nd;
+	const double tol = sz == 8 ? 64.0 * 2.220446049250313e-16 : 64.0 * 1.1920929e-7;
+	for (size_t g = 0; g < c->n_rays; ++g) {
+		double tau[GRAYHIP_MAX_FREQ];
+		for (int i = 0; i < nf; ++i) {
+			const size_t k = g * (size_t)nd + 8 + (size_t)nf + (size_t)i;
+			tau[i] = sz == 8 ? ((const double *)data)[k] : (double)((const float *)data)[k];
+		}
+		for (int i = 1; i < nf; ++i) {
+			const double lhs = tau[i] * d[0], rhs = tau[0] * d[i];
+			if (fabs(lhs - rhs) > tol * fmax(fabs(lhs), fabs(rhs)))
+				return fail(c, GRAYHIP_EINVAL,
+				            "ray %zu: optical depth of channel %d (%.17g) is not "
+				            "tau_0 (nu_0/nu_%d)^(5/2) = %.17g; with the disk and uniform "
+				            "fluids every channel's depth is one shared integral",
+				            g, i, tau[i], i, rhs / d[0]);
+		}
+	}
+	return GRAYHIP_OK;
+}
+
 extern "C" int grayhip_write(grayhip_ctx *c, const void *data)
 {
 	if (!c || !data)
 		return fail(c, GRAYHIP_EINVAL, "null argument");
 	const int sz = c->cfg.precision;
+	int rc = check_depths(c, data);
+	if (rc)
+		return rc;
 	HCK(c, hipSetDevice(c->cfg.device));
 	HCK(c, hipMemcpyAsync(c->d_tmp, data, c->n_rays * (size_t)c->nd * sz, hipMemcpyHostToDevice,
 	                      c->stream));

@@ -236,7 +236,9 @@ extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q
 {
 	if (!v || !q || !id)
 		return vfail(v, GRAYHIP_EINVAL, "null argument");
-	if (q->w_rays == 0 || q->h_rays == 0 || q->w_rays * q->h_rays > (size_t)INT32_MAX)
+	/* each side bounded first, so the product cannot wrap */
+	if (q->w_rays == 0 || q->h_rays == 0 || q->w_rays > (size_t)INT32_MAX ||
+	    q->h_rays > (size_t)INT32_MAX || q->w_rays * q->h_rays > (size_t)INT32_MAX)
 		return vfail(v, GRAYHIP_EINVAL, "bad resolution");
 	if (q->n_freq < 0 || q->n_freq > SRV_NF)
 		return vfail(v, GRAYHIP_EINVAL, "n_freq must be in [0, %d]", SRV_NF);

@@ -51,20 +51,28 @@ __device__ __forceinline__ void sample_f64(const S *const lev[2], int base,
 	}
 }
 
+/* k_mu = g_{mu nu} u^nu with the interpolated metric (sim-org/dyst.cl:55-81,
+ * with g_zz read from g_zz); s = (q, u). */
 template <class S>
-__device__ __forceinline__ double grid_getuu_f64(const GridView<S> &G, const double *s)
+__device__ __forceinline__ void grid_down_f64(const GridView<S> &G, const double *s, double k[4])
 {
 	Stencil<double> st;
 	make_stencil<double, S>(G, s[0], s[1], s[2], s[3], st);
 	double m[10];
 	sample_f64<10, 10, S>(G.met, 0, st, m);
 	const double u[4] = {s[4], s[5], s[6], s[7]};
-	double k[4];
 #pragma unroll
 	for (int a = 0; a < 4; ++a)
 		k[a] = ((m[sym(a, 0)] * u[0] + m[sym(a, 1)] * u[1]) + m[sym(a, 2)] * u[2]) +
 		       m[sym(a, 3)] * u[3];
-	return ((u[0] * k[0] + u[1] * k[1]) + u[2] * k[2]) + u[3] * k[3];
+}
+
+template <class S>
+__device__ __forceinline__ double grid_getuu_f64(const GridView<S> &G, const double *s)
+{
+	double k[4];
+	grid_down_f64<S>(G, s, k);
+	return ((s[4] * k[0] + s[5] * k[1]) + s[6] * k[2]) + s[7] * k[3];
 }
 
 } // namespace gray

@@ -111,6 +111,23 @@ __global__ __launch_bounds__(256) void k_info_f64(RayParams p, const double *__r
 	info[g] = getuu(p.a, q, u);
 }
 
+/* Covariant view: out[g][8] = (q, k_mu) with k = down(q, u)
+ * (sim-org/KS.cl:91-114), the oracle's og_down sequence. */
+__global__ __launch_bounds__(256) void k_covariant(double a, const double *__restrict__ data,
+                                                   double *__restrict__ out, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	V4<double> q = {data[g], data[n_rays + g], data[2 * n_rays + g], data[3 * n_rays + g]};
+	V4<double> u = {data[4 * n_rays + g], data[5 * n_rays + g], data[6 * n_rays + g],
+	                data[7 * n_rays + g]};
+	V4<double> k = down(a, q, u);
+	double *o = out + (size_t)8 * g;
+	o[0] = q.t; o[1] = q.x; o[2] = q.y; o[3] = q.z;
+	o[4] = k.t; o[5] = k.x; o[6] = k.y; o[7] = k.z;
+}
+
 /* Boyer-Lindquist view of the final states (sim-org/KS.cl:63-149) in the
  * spheroidal convention of the metric, x + i y = (r + i a) e^{i phi}
  * sin(theta), and with u^r in the t and phi corrections (the derivative of
@@ -174,6 +191,16 @@ hipError_t launch_to_bl(double a, double r_match, const double *data, double *ou
 		return hipSuccess;
 	hipLaunchKernelGGL(k_to_bl, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, a, r_match, data,
 	                   out, n_rays);
+	return hipGetLastError();
+}
+
+hipError_t launch_covariant(double a, const double *data, double *out, size_t n_rays,
+                            hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(k_covariant, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st, a, data, out,
+	                   n_rays);
 	return hipGetLastError();
 }
 

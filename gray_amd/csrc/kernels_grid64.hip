@@ -291,6 +291,29 @@ __global__ __launch_bounds__(256) void k_grid_info_f64(GridView<S> G, const doub
 	info[g] = grid_getuu_f64<S>(G, s);
 }
 
+/* Covariant view: out[g][8] = (q, k_mu), k = g(q) u with the interpolated
+ * metric. */
+template <class S>
+__global__ __launch_bounds__(256) void k_grid_covariant(GridView<S> G, const double *__restrict__ data,
+                                                        double *__restrict__ out, size_t n_rays)
+{
+	size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= n_rays)
+		return;
+	double s[8];
+#pragma unroll
+	for (int i = 0; i < 8; ++i)
+		s[i] = data[(size_t)i * n_rays + g];
+	double k[4];
+	grid_down_f64<S>(G, s, k);
+	double *o = out + (size_t)8 * g;
+#pragma unroll
+	for (int i = 0; i < 4; ++i) {
+		o[i] = s[i];
+		o[4 + i] = k[i];
+	}
+}
+
 /* Sample of one field at given events (test and tool helper). */
 template <class S>
 __global__ void k_grid_sample_f64(GridView<S> G, const double *__restrict__ ev, int which, int comp,
@@ -371,6 +394,20 @@ hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *
 	else
 		hipLaunchKernelGGL(k_grid_info_f64<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
 		                   gd.view<double>(), data, info, n_rays);
+	return hipGetLastError();
+}
+
+hipError_t launch_grid_covariant(const GridDesc &gd, const double *data, double *out, size_t n_rays,
+                                 hipStream_t st)
+{
+	if (n_rays == 0)
+		return hipSuccess;
+	if (gd.dtype == 4)
+		hipLaunchKernelGGL(k_grid_covariant<float>, dim3(blocks_for(n_rays, 256)), dim3(256), 0, st,
+		                   gd.view<float>(), data, out, n_rays);
+	else
+		hipLaunchKernelGGL(k_grid_covariant<double>, dim3(blocks_for(n_rays, 256)), dim3(256), 0,
+		                   st, gd.view<double>(), data, out, n_rays);
 	return hipGetLastError();
 }
 

@@ -44,6 +44,9 @@ hipError_t launch_info_f64(const RayParams &p, const double *data, double *info,
                            hipStream_t st);
 hipError_t launch_to_bl(double a, double r_match, const double *data, double *out, size_t n_rays,
                         hipStream_t st);
+/* (q, k_mu = g(q) u) per ray, AoS [n_rays][8] doubles. */
+hipError_t launch_covariant(double a, const double *data, double *out, size_t n_rays,
+                            hipStream_t st);
 
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
@@ -60,6 +63,8 @@ hipError_t launch_grid_tiles_f64(const GridDesc &gd, const RtThermal &rt, const 
                                  int H, bool bricks, hipStream_t st);
 hipError_t launch_grid_info_f64(const GridDesc &gd, const double *data, double *info, size_t n_rays,
                                 hipStream_t st);
+hipError_t launch_grid_covariant(const GridDesc &gd, const double *data, double *out, size_t n_rays,
+                                 hipStream_t st);
 hipError_t launch_grid_sample_f64(const GridDesc &gd, const double *ev, int which, int comp,
                                   double *out, int n, hipStream_t st);
 /* FP32 gridded form, n_dump intervals of every ray in one launch (8x8-ray

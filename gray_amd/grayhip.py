@@ -21,7 +21,7 @@ EXPORTS = (
     "grayhip_create", "grayhip_destroy", "grayhip_icond", "grayhip_evolve",
     "grayhip_render", "grayhip_read", "grayhip_read_f64", "grayhip_write",
     "grayhip_read_steps", "grayhip_stats", "grayhip_pass_steps", "grayhip_kernel_time",
-    "grayhip_n_data", "grayhip_read_bl",
+    "grayhip_n_data", "grayhip_read_bl", "grayhip_read_covariant",
     "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
     "grayhip_spacetime_create", "grayhip_spacetime_destroy", "grayhip_spacetime_upload",
     "grayhip_spacetime_shift", "grayhip_spacetime_set_frozen", "grayhip_spacetime_bytes",
@@ -92,6 +92,7 @@ def lib():
     L.grayhip_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.grayhip_n_data.argtypes = [vp]
     L.grayhip_read_bl.argtypes = [vp, C.c_double, vp]
+    L.grayhip_read_covariant.argtypes = [vp, vp]
     L.grayhip_last_error.argtypes = [vp]
     L.grayhip_last_error.restype = C.c_char_p
     L.grayhip_global_error.argtypes = []
@@ -277,6 +278,12 @@ class Render:
         """Boyer-Lindquist states [n_rays, 8]: t, r, theta, phi, u^t, u^r, u^theta, u^phi."""
         out = np.empty((self.n_rays, 8), dtype=np.float64)
         self._check(lib().grayhip_read_bl(self._h, r_match, _ptr(out)), "read_bl")
+        return out
+
+    def read_covariant(self):
+        """States [n_rays, 8]: t, x, y, z, k_t, k_x, k_y, k_z (k = g u)."""
+        out = np.empty((self.n_rays, 8), dtype=np.float64)
+        self._check(lib().grayhip_read_covariant(self._h, _ptr(out)), "read_covariant")
         return out
 
     def write(self, data):

@@ -85,14 +85,18 @@ void gray_grid_destroy(gray_grid *g)
 {
 	if (!g)
 		return;
+	/* a prefetch thread still reading uses names, path and the pf_
+	 * buffers: it ends before anything is freed */
+	if (g->pf_active) {
+		pthread_join(g->pf_thread, NULL);
+		g->pf_active = 0;
+	}
 	if (g->ctx)
 		grayhip_destroy(g->ctx);
 	if (g->st)
 		grayhip_spacetime_destroy(g->st);
 	free(g->names);
 	free(g->times);
-	if (g->pf_active)
-		pthread_join(g->pf_thread, NULL);
 	free(g->gam);
 	free(g->met);
 	free(g->rho);

@@ -160,31 +160,38 @@ static int cmp_d(const void *a, const void *b)
 	return (x > y) - (x < y);
 }
 
-int gray_serve_run(const gray_serve_opts *o, gray_serve_report *rep, double *latency_ms)
+int gray_serve_run_items(const gray_serve_opts *o, const gray_serve_item *it, int n,
+                         gray_serve_report *rep, double *latency_ms, double *const *results)
 {
-	const int n = o->n_requests;
-	gray_serve_item *it = calloc((size_t)n, sizeof(*it));
-	double *t_done = calloc((size_t)n, sizeof(double));
-	int64_t *ids = calloc((size_t)n, sizeof(int64_t));
+	double *t_done = calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+	int64_t *ids = calloc((size_t)(n > 0 ? n : 1), sizeof(int64_t));
 	grayhip_spacetime *grids[2] = {NULL, NULL};
 	grayhip_server *srv = NULL;
 	int rc = -1;
 	memset(rep, 0, sizeof(*rep));
-	if (!it || !t_done || !ids)
+	if (n <= 0 || !t_done || !ids)
 		goto out;
-	gray_serve_stream(o, it);
-	for (int g = 0; g < 2; ++g)
-		if (make_grid(o->grid_n[g], o->device, &grids[g])) {
+	for (int k = 1; k < n; ++k)
+		if (it[k].arrival_s < it[k - 1].arrival_s) {
+			fprintf(stderr, "gray-serve: arrivals must be in time order\n");
+			goto out;
+		}
+	for (int g = 0; g < 2; ++g) {
+		int used = 0;
+		for (int k = 0; k < n; ++k)
+			used |= it[k].grid == g;
+		if (used && make_grid(o->grid_n[g], o->device, &grids[g])) {
 			fprintf(stderr, "gray-serve: cannot build grid %d\n", g);
 			goto out;
 		}
+	}
 	grayhip_server_config sc = {o->device, o->slots, o->hbm_budget, o->epoch_steps, 4096};
 	if (grayhip_server_create(&sc, &srv)) {
 		fprintf(stderr, "gray-serve: %s\n", grayhip_server_error(NULL));
 		goto out;
 	}
 	int next = 0, n_done = 0;
-	const double t0 = now_s();
+	const double t0 = now_s() - it[0].arrival_s;
 	while (n_done < n) {
 		double t = now_s() - t0;
 		for (; next < n && it[next].arrival_s <= t; ++next) {
@@ -203,13 +210,18 @@ int gray_serve_run(const gray_serve_opts *o, gray_serve_report *rep, double *lat
 		int64_t id;
 		while (grayhip_server_next_done(srv, &id) == 0) {
 			double td = now_s() - t0;
-			/* ids are issued in submission (= stream) order from 1 */
+			/* ids are issued in submission (= list) order */
 			int k = (int)(id - ids[0]);
+			double *res = NULL;
 			if (k >= 0 && k < n) {
 				t_done[k] = td;
 				rep->rays += (int64_t)it[k].side * it[k].side;
+				res = results ? results[k] : NULL;
 			}
-			grayhip_server_result(srv, id, NULL, NULL, NULL);
+			if (grayhip_server_result(srv, id, res, NULL, NULL)) {
+				fprintf(stderr, "gray-serve: result: %s\n", grayhip_server_error(srv));
+				goto out;
+			}
 			++n_done;
 			if (o->verbose) {
 				printf("request %d (%dx%d %s) done: %.1f ms\n", k, it[k].side, it[k].side,
@@ -254,8 +266,18 @@ out:
 	grayhip_server_destroy(srv);
 	grayhip_spacetime_destroy(grids[0]);
 	grayhip_spacetime_destroy(grids[1]);
-	free(it);
 	free(t_done);
 	free(ids);
+	return rc;
+}
+
+int gray_serve_run(const gray_serve_opts *o, gray_serve_report *rep, double *latency_ms)
+{
+	gray_serve_item *it = calloc((size_t)(o->n_requests > 0 ? o->n_requests : 1), sizeof(*it));
+	if (!it)
+		return -1;
+	gray_serve_stream(o, it);
+	int rc = gray_serve_run_items(o, it, o->n_requests, rep, latency_ms, NULL);
+	free(it);
 	return rc;
 }

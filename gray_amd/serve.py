@@ -5,7 +5,7 @@ import numpy as np
 
 from . import job as _job
 
-EXPORTS = ("gray_serve_defaults", "gray_serve_run", "gray_serve_stream")
+EXPORTS = ("gray_serve_defaults", "gray_serve_run", "gray_serve_stream", "gray_serve_run_items")
 
 
 class Opts(C.Structure):
@@ -52,6 +52,15 @@ class Item(C.Structure):
 
 _set = False
 
+# The request loop gives gridded requests no transfer channels (the pool
+# carries channels on analytic-Kerr requests only).
+GRID_CHANNELS = False
+
+
+def served_channels(item):
+    """Channels a served request of the stream carries."""
+    return item.n_freq if (item.grid < 0 or GRID_CHANNELS) else 0
+
 
 def lib():
     global _set
@@ -61,6 +70,8 @@ def lib():
         L.gray_serve_defaults.restype = None
         L.gray_serve_run.argtypes = [C.POINTER(Opts), C.POINTER(Report), C.c_void_p]
         L.gray_serve_stream.argtypes = [C.POINTER(Opts), C.c_void_p]
+        L.gray_serve_run_items.argtypes = [C.POINTER(Opts), C.c_void_p, C.c_int,
+                                           C.POINTER(Report), C.c_void_p, C.c_void_p]
         _set = True
     return L
 
@@ -92,3 +103,26 @@ def run(opts):
     d = {f: getattr(rep, f) for f, _ in Report._fields_}
     d["ray_steps"] = list(rep.ray_steps)
     return d, lat
+
+
+def run_items(opts, items, keep_results=False):
+    """Serve the given items (a list of Item, arrivals in time order).
+    Returns (report dict, per-request latency ms, results or None), results
+    being each request's AoS FP64 rays [side*side, 8 + 2 n_freq]."""
+    n = len(items)
+    arr = (Item * n)(*items)
+    rep = Report()
+    lat = np.zeros(n)
+    res = None
+    ptrs = None
+    if keep_results:
+        res = [np.zeros((it.side * it.side, 8 + 2 * served_channels(it))) for it in items]
+        ptrs = (C.c_void_p * n)(*[r.ctypes.data for r in res])
+    rc = lib().gray_serve_run_items(C.byref(opts), C.cast(arr, C.c_void_p), n, C.byref(rep),
+                                    C.c_void_p(lat.ctypes.data),
+                                    C.cast(ptrs, C.c_void_p) if ptrs is not None else None)
+    if rc != 0:
+        raise RuntimeError("gray_serve_run_items failed")
+    d = {f: getattr(rep, f) for f, _ in Report._fields_}
+    d["ray_steps"] = list(rep.ray_steps)
+    return d, lat, res

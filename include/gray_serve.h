@@ -61,6 +61,14 @@ typedef struct {
 
 int  gray_serve_stream(const gray_serve_opts *o, gray_serve_item *items);
 
+/* Serve a given list of n requests (arrival_s in time order; the first is
+ * submitted at once and the others at their offsets from it) through one
+ * pool, as gray_serve_run does for the seeded stream.  results (may be
+ * NULL, as may any entry) receives request k's rays, AoS [side*side][8 +
+ * 2 n_freq] FP64, in results[k]. */
+int  gray_serve_run_items(const gray_serve_opts *o, const gray_serve_item *items, int n,
+                          gray_serve_report *rep, double *latency_ms, double *const *results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -153,7 +153,18 @@ int  grayhip_read_f64(grayhip_ctx *ctx, double *data, double *info);
  * uses u^r in the t and phi corrections.  Rays inside the horizon give NaN. */
 int  grayhip_read_bl(grayhip_ctx *ctx, double r_match, double *out);
 
-/* Upload AoS states (same layout as grayhip_read) to restart from a dump. */
+/* Covariant view of the states: AoS [H][W][8] doubles (t, x, y, z, k_t,
+ * k_x, k_y, k_z), k_mu = g_{mu nu}(q) u^nu, the output the reference
+ * obtains with down() (sim-org/KS.cl:91-114 on analytic Kerr-Schild,
+ * dyst.cl:55-81 with the interpolated metric on a numerical spacetime);
+ * computed in FP64 from the stored state in either form. */
+int  grayhip_read_covariant(grayhip_ctx *ctx, double *out);
+
+/* Upload AoS states (same layout as grayhip_read) to restart from a dump.
+ * With the disk or uniform fluid every channel's optical depth is one
+ * shared integral T (tau_i = 0.1 nu_i^-5/2 T, as those fluids absorb): a
+ * state whose depths are not in that proportion (to 64 rounding errors of
+ * the real size) is rejected with GRAYHIP_EINVAL. */
 int  grayhip_write(grayhip_ctx *ctx, const void *data);
 
 /* Per-ray count of RK4 steps executed since icond (int64[n_rays]). */

@@ -71,3 +71,34 @@ def test_plane_inside_the_shadow():
     r = np.sqrt(np.sum(got[0][:, 1:4] ** 2, axis=1))
     assert (r < 2.0).all()
     assert (got[2] < 64 * 1024).all()
+
+
+def test_restart_keeps_shared_depths_and_rejects_others():
+    """grayhip_write then render continues a dump exactly as an uninterrupted
+    run (plain FP64 form); depths not in the proportion of the disk's
+    absorption law are refused."""
+    a, nu, w = 0.9375, (1.0, 2.67, 4.0), 12
+    with G.Render(w, w, a, precision=G.FP64, nu=nu) as r:
+        r.icond(48.0, 48.0, 1000.0, 60.0, 0.0)
+        r.render(-32.0, 1024, 40)
+        mid, _ = r.read()
+        r.render(-32.0, 1024, 88)
+        ref, ref_info = r.read_f64()
+    assert (mid[:, 11:14] > 0).any()
+    with G.Render(w, w, a, precision=G.FP64, nu=nu) as r:
+        r.write(mid)
+        r.render(-32.0, 1024, 88)
+        got, got_info = r.read_f64()
+        assert np.array_equal(got, ref) and np.array_equal(got_info, ref_info)
+        bad = mid.copy()
+        k = int(np.argmax(bad[:, 11]))
+        bad[k, 12] *= 1.001
+        with pytest.raises(G.GrayHipError, match="optical depth"):
+            r.write(bad)
+    with G.Render(w, w, a, precision=G.FP32, nu=nu) as r:
+        r.write(mid.astype(np.float32))
+        r.render(-32.0, 1024, 88)
+        got32, _ = r.read_f64()
+    I0, I1 = ref[:, 8:11], got32[:, 8:11]
+    l1 = np.abs(I1 - I0).sum(0) / np.abs(I0).sum(0)
+    assert (l1 <= 1e-4).all(), l1

@@ -2,14 +2,15 @@
 in seconds, plus the size-independent image properties of the whole frame.
 
   * analytic Kerr, 1024^2 rays, disk transfer with 4 channels (the headline
-    workload): the plain FP64 form against the oracle on every 64th row and
-    column (geodesic state bit for bit, I and tau to 1e-10 relative); the
+    workload): the plain FP64 form against the oracle on every 16th row and
+    column, 4096 rays (geodesic state bit for bit, I and tau to 1e-10
+    relative); the
     FP32 form against the plain FP64 form on the whole image (relative L1
     per channel <= 1e-4, peak-pixel error <= 5e-2, null-constraint drift of
     escaping rays <= 1e-3 and <= 1e-5);
   * 128^3 boosted Kerr-Schild grid with two time levels, 512^2 rays: the
-    plain FP64 form against the grid oracle on every 32nd row and column
-    (state, step counts and g(u,u) bit for bit).
+    plain FP64 form against the grid oracle on every 16th row and column,
+    1024 rays (state, step counts and g(u,u) bit for bit).
 """
 import numpy as np
 import pytest
@@ -36,12 +37,12 @@ def test_kerr_1024_fp64_subset_matches_oracle_and_fp32_image():
     d64, i64 = kerr_render(G.FP64)
     p = O.make_params(A, nu=NUS)
     data, _ = O.icond_all(p, 1024, 1024, 64.0, 64.0, 1000.0, 60.0, 0.0)
-    sub = np.ascontiguousarray(data.reshape(1024, 1024, -1)[::64, ::64].reshape(256, -1))
+    sub = np.ascontiguousarray(data.reshape(1024, 1024, -1)[::16, ::16].reshape(4096, -1))
     O.evolve_subset(p, sub, 1, -32.0, 1024, 128)
-    got = d64.reshape(1024, 1024, -1)[::64, ::64].reshape(256, -1)
+    got = d64.reshape(1024, 1024, -1)[::16, ::16].reshape(4096, -1)
     assert np.array_equal(got[:, :8], sub[:, :8])
     rt0, rt1 = sub[:, 8:], got[:, 8:]
-    assert (rt0[:, :4] > 0).sum() > 20
+    assert (rt0[:, :4] > 0).sum() > 300
     assert np.all(np.abs(rt1 - rt0) <= 1e-10 * np.abs(rt0))
 
     d32, i32 = kerr_render(G.FP32)
@@ -74,7 +75,7 @@ def test_grid_128_fp64_subset_matches_oracle():
                          [g for g, _ in lv], [m for _, m in lv],
                          times=times, t_off=1.0)
     data, info = og.icond_all(512, 512, 32.0, 32.0, 35.0, 45.0, 0.0)
-    pick = (np.arange(512)[::32, None] * 512 + np.arange(512)[None, ::32]).ravel()
+    pick = (np.arange(512)[::16, None] * 512 + np.arange(512)[None, ::16]).ravel()
     d0, i0 = np.ascontiguousarray(data[pick]), np.ascontiguousarray(info[pick])
     s0 = np.zeros(len(pick), dtype=np.int64)
     for _ in range(4):

@@ -289,3 +289,22 @@ def test_bricks_do_not_change_results(dtype, precision, monkeypatch):
     assert np.array_equal(s0, s1)
     assert np.array_equal(d0, d1)
     assert np.array_equal(i0, i1)
+
+
+@pytest.mark.parametrize("dtype", [4, 8])
+def test_covariant_view_bitwise(dtype):
+    """k_mu = g(q) u with the interpolated metric (dyst.cl down) of the final
+    states of a two-level render: the grid oracle's, bit for bit."""
+    n, times = 33, (0.0, 1.0)
+    xs, lv = make_grid(n, dtype, times=times)
+    st = device_grid(n, dtype, lv, times)
+    with G.Render(11, 9, precision=G.FP64, spacetime=st, t_init=1.0) as r:
+        r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+        r.render(-32.0, 1024, 2)
+        d1, _ = r.read_f64()
+        cov = r.read_covariant()
+    st.close()
+    og = oracle_grid(n, lv, times, t_off=1.0)
+    want = np.array([np.concatenate([d1[g, :4], og.down(d1[g, :4], d1[g, 4:8])])
+                     for g in range(len(d1))])
+    assert np.array_equal(cov, want)

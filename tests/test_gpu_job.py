@@ -32,7 +32,8 @@ def test_launcher_kerr_dumps_match_oracle(tmp_path):
         assert np.array_equal(s.reshape(-1, 12)[:, :8], data[:, :8]), i
         rel = np.abs(s.reshape(-1, 12) - data) / np.maximum(np.abs(data), 1e-300)
         assert rel.max() <= 1e-10
-        assert np.array_equal(d.ravel(), info) or i > 0
+        # g(u,u) of every dump: the oracle's info after the same intervals
+        assert d.shape == (16, 24) and np.array_equal(d.ravel(), info), i
         O.evolve_all(p, data, info, -32.0, 1024)
 
 

@@ -183,3 +183,23 @@ def test_boyer_lindquist_view_matches_oracle():
         with G.Render(4, 4, A, precision=G.FP64) as r2:
             r2.icond(8.0, 8.0, 1000.0, 60.0, 0.0)
             r2.read_bl(1.0)            # r_match inside the horizon
+
+
+def test_config1_covariant_output_matches_oracle():
+    """The 64^2 FP64 job with its stated output (t, x, y, z, k_t, k_x, k_y,
+    k_z): k_mu = down(q, u) of the final states equals the oracle's, bit
+    for bit, and k.k = g(u,u) stays null."""
+    a = 0.9375
+    p = O.make_params(a)
+    d0, i0, s0 = O.render(p, 64, 64, 64.0, 64.0, 1000.0, 60.0, 0.0)
+    with G.Render(64, 64, a, precision=G.FP64) as r:
+        r.icond(64.0, 64.0, 1000.0, 60.0, 0.0)
+        r.render(-32.0, 1024, 128)
+        cov = r.read_covariant()
+        d1, _ = r.read_f64()
+    assert np.array_equal(d1, d0)
+    want = np.array([np.concatenate([d0[g, :4], O.down(a, d0[g, :4], d0[g, 4:8])])
+                     for g in range(64 * 64)])
+    assert np.array_equal(cov, want)
+    kk = np.einsum("ij,ij->i", cov[:, 4:], d0[:, 4:8])
+    assert np.abs(kk).max() <= 1e-6

@@ -56,3 +56,40 @@ def test_precision_form_vs_oracle_subset():
     I0, I1 = sub[:, 8:12], got[:, 8:12]
     l1 = np.abs(I1 - I0).sum(0) / np.abs(I0).sum(0)
     assert (l1 <= 1e-4).all(), l1
+
+
+def test_precision_form_2048_full_size():
+    """The precision workload at its full size: 2048^2 rays, a = 0.998,
+    i = 85 deg, 4 channels.  FP32 form against the plain FP64 form on the
+    whole image (the bars above), and both against the oracle on every 64th
+    row and column (1024 rays): the plain FP64 form bit for bit in the
+    geodesic state and step counts, I and tau to 1e-10 relative; the FP32
+    form's image L1 on that subset <= 1e-4."""
+    w = 2048
+    d64, i64, s64, _ = render(G.FP64, w)
+    d32, i32, s32, nfb = render(G.FP32, w)
+    I64, I32 = d64[:, 8:12], d32[:, 8:12]
+    l1 = np.abs(I32 - I64).sum(0) / np.abs(I64).sum(0)
+    peak = np.abs(I32 - I64).max() / np.abs(I64).max()
+    esc = np.sqrt(np.sum(d64[:, 1:4] ** 2, axis=1)) > 999.0
+    drift64, drift32 = np.abs(i64[esc]).max(), np.abs(i32[esc]).max()
+    print(f"2048^2: L1 {l1}, peak {peak:.3e}, null drift fp64 {drift64:.2e} "
+          f"fp32 {drift32:.2e}, fallback rays {nfb}, escaping {esc.sum()}", flush=True)
+    assert (l1 <= 1e-4).all(), l1
+    assert peak <= 5e-2, peak
+    assert esc.sum() > 3_000_000
+    assert drift64 <= 1e-5 and drift32 <= 1e-3, (drift32, drift64)
+    assert nfb > 1000
+
+    p = O.make_params(A, nu=NUS)
+    data, _ = O.icond_all(p, w, w, 64.0, 64.0, 1000.0, INC, 0.0)
+    pick = (np.arange(w)[::64, None] * w + np.arange(w)[None, ::64]).ravel()
+    sub = np.ascontiguousarray(data[pick])
+    steps0 = O.evolve_subset(p, sub, 1, -32.0, 1024, 128)
+    assert steps0 == s64[pick].sum()
+    assert np.array_equal(d64[pick, :8], sub[:, :8])
+    rt0, rt1 = sub[:, 8:], d64[pick, 8:]
+    assert (rt0[:, :4] > 0).sum() > 40
+    assert np.all(np.abs(rt1 - rt0) <= 1e-10 * np.abs(rt0))
+    l1o = np.abs(d32[pick, 8:12] - sub[:, 8:12]).sum(0) / np.abs(sub[:, 8:12]).sum(0)
+    assert (l1o <= 1e-4).all(), l1o
