@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "launch.h"
@@ -149,11 +150,10 @@ extern "C" void grayhip_destroy(grayhip_ctx *ctx)
 	release(ctx);
 }
 
-extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
+/* Arguments of a render, checked before anything is allocated (messages
+ * go to the calling thread's error string). */
+static int check_config(const grayhip_config *cfg)
 {
-	if (!cfg || !out)
-		return fail(nullptr, GRAYHIP_EINVAL, "null argument");
-	*out = nullptr;
 	if (cfg->w_rays == 0 || cfg->h_rays == 0)
 		return fail(nullptr, GRAYHIP_EINVAL, "w_rays and h_rays must be positive");
 	if (cfg->w_rays > INT_MAX || cfg->h_rays > INT_MAX ||
@@ -189,14 +189,12 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 	if (cfg->device < 0 || cfg->device >= ndev)
 		return fail(nullptr, GRAYHIP_EINVAL, "device %d out of range (%d visible)", cfg->device, ndev);
 
-	grayhip_ctx *c = (grayhip_ctx *)calloc(1, sizeof(grayhip_ctx));
-	if (!c)
-		return fail(nullptr, GRAYHIP_ENOMEM, "host allocation failed");
-	c->cfg = *cfg;
-	c->n_rays = cfg->w_rays * cfg->h_rays;
-	c->nd = 8 + 2 * cfg->n_freq;
+	return GRAYHIP_OK;
+}
 
-	RayParams &p = c->p;
+/* Physics and image size of a render (the camera comes with icond). */
+static void fill_params(const grayhip_config *cfg, RayParams &p)
+{
 	memset(&p, 0, sizeof(p));
 	p.a = cfg->a_spin;
 	p.r_h = 1.0 + sqrt(1.0 - p.a * p.a);
@@ -211,6 +209,41 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 		p.nu[i] = cfg->nu[i];
 	p.w_rays = (int)cfg->w_rays;
 	p.h_rays = (int)cfg->h_rays;
+}
+
+/* Camera of sim-org/icond.opts:2-6, trigonometry in FP64 on the host. */
+static void fill_camera(RayParams &p, double w_img, double h_img, double r_obs, double i_obs,
+                        double j_obs)
+{
+	const double d2r = M_PI / 180.0;
+	p.w_img = w_img;
+	p.h_img = h_img;
+	p.r_obs = r_obs;
+	p.si = sin(d2r * i_obs);
+	p.ci = cos(d2r * i_obs);
+	p.sj = sin(d2r * j_obs);
+	p.cj = cos(d2r * j_obs);
+}
+
+extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
+{
+	if (!cfg || !out)
+		return fail(nullptr, GRAYHIP_EINVAL, "null argument");
+	*out = nullptr;
+	{
+		int rc0 = check_config(cfg);
+		if (rc0)
+			return rc0;
+	}
+
+	grayhip_ctx *c = (grayhip_ctx *)calloc(1, sizeof(grayhip_ctx));
+	if (!c)
+		return fail(nullptr, GRAYHIP_ENOMEM, "host allocation failed");
+	c->cfg = *cfg;
+	c->n_rays = cfg->w_rays * cfg->h_rays;
+	c->nd = 8 + 2 * cfg->n_freq;
+
+	fill_params(cfg, c->p);
 	c->st = cfg->spacetime;
 	rt_thermal_init(&c->rt, cfg->m_adm > 0.0 ? cfg->m_adm : 4.3e6,
 	                c->st ? cfg->n_freq : 0, cfg->nu);
@@ -278,14 +311,7 @@ extern "C" int grayhip_icond(grayhip_ctx *c, double w_img, double h_img, double 
 		return fail(nullptr, GRAYHIP_EINVAL, "null context");
 	if (!(r_obs > 0.0))
 		return fail(c, GRAYHIP_EINVAL, "r_obs must be positive");
-	const double d2r = M_PI / 180.0;
-	c->p.w_img = w_img;
-	c->p.h_img = h_img;
-	c->p.r_obs = r_obs;
-	c->p.si = sin(d2r * i_obs);
-	c->p.ci = cos(d2r * i_obs);
-	c->p.sj = sin(d2r * j_obs);
-	c->p.cj = cos(d2r * j_obs);
+	fill_camera(c->p, w_img, h_img, r_obs, i_obs, j_obs);
 	HCK(c, hipSetDevice(c->cfg.device));
 	if (c->st && c->rt.n_freq > 0 && !grayhip_spacetime_has_rho(c->st))
 		return fail(c, GRAYHIP_ESTATE, "transfer needs the density of every level in use "
@@ -716,5 +742,109 @@ extern "C" int grayhip_kernel_time(grayhip_ctx *c, double *ns, int *launches)
 		*ns = c->kern_ns;
 	if (launches)
 		*launches = c->kern_launches;
+	return GRAYHIP_OK;
+}
+
+/* ---------------------------------------------------------------------
+ * grayhip_render_static: icond + render for callers that cannot allocate.
+ *
+ * The FP32 form's scratch (pass counters and the hand-over queue of the
+ * FP64 re-integration) lives in `scratch`, a file-scope device array of
+ * fixed size, instead of memory a context allocates.  Each GPU holds its
+ * own copy until the process ends; its device address is looked up once
+ * per device (hipGetSymbolAddress with that device current) and only that
+ * address is handed to kernels.  Every call on one GPU shares the array:
+ * calls that overlap on one GPU race, and the counters are cleared by a
+ * kernel at the start of every call.
+ * --------------------------------------------------------------------- */
+
+constexpr size_t SCRATCH_FB_OFF = 256;   /* Counters first, queue after */
+constexpr size_t SCRATCH_BYTES =
+    SCRATCH_FB_OFF + (size_t)GRAYHIP_STATIC_MAX_RAYS * sizeof(Fallback);
+static_assert(sizeof(Counters) <= SCRATCH_FB_OFF, "counters overlap the queue");
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int STATIC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[STATIC_MAX_DEVICES];
+
+/* Device address of this device's copy of `scratch` (device current). */
+static int static_scratch(int device, unsigned char **out)
+{
+	if (device < 0 || device >= STATIC_MAX_DEVICES)
+		return fail(nullptr, GRAYHIP_EINVAL, "device %d beyond the %d the library serves",
+		            device, STATIC_MAX_DEVICES);
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_ptr[device]) {
+		void *ptr = nullptr;
+		hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr)
+			return fail(nullptr, GRAYHIP_EDEVICE, "hipGetSymbolAddress(scratch): %s",
+			            hipGetErrorString(e));
+		g_scratch_ptr[device] = ptr;
+	}
+	*out = (unsigned char *)g_scratch_ptr[device];
+	return GRAYHIP_OK;
+}
+
+#define SCKG(expr)                                                                        \
+	do {                                                                              \
+		hipError_t e_ = (expr);                                                   \
+		if (e_ != hipSuccess)                                                     \
+			return fail(nullptr, GRAYHIP_EDEVICE, "%s: %s", #expr,            \
+			            hipGetErrorString(e_));                               \
+	} while (0)
+
+extern "C" int grayhip_render_static(const grayhip_config *cfg, double w_img, double h_img,
+                                     double r_obs, double i_obs, double j_obs, double dt,
+                                     size_t n_sub, size_t n_dump, double *data, double *info,
+                                     int64_t *steps, void *stream)
+{
+	if (!cfg || !data || !info || !steps)
+		return fail(nullptr, GRAYHIP_EINVAL, "null argument");
+	int rc = check_config(cfg);
+	if (rc)
+		return rc;
+	if (cfg->spacetime)
+		return fail(nullptr, GRAYHIP_EINVAL, "grayhip_render_static renders analytic Kerr only");
+	const size_t n = cfg->w_rays * cfg->h_rays;
+	if (n > (size_t)GRAYHIP_STATIC_MAX_RAYS)
+		return fail(nullptr, GRAYHIP_EINVAL, "%zu rays exceed the static scratch's %d",
+		            n, GRAYHIP_STATIC_MAX_RAYS);
+	if (!(r_obs > 0.0))
+		return fail(nullptr, GRAYHIP_EINVAL, "r_obs must be positive");
+	StepRule sr;
+	if ((rc = make_rule(nullptr, dt, n_sub, n_dump, &sr)))
+		return rc;
+	RayParams p;
+	fill_params(cfg, p);
+	fill_camera(p, w_img, h_img, r_obs, i_obs, j_obs);
+	hipStream_t st = (hipStream_t)stream;
+	SCKG(hipSetDevice(cfg->device));
+	unsigned char *base = nullptr;
+	if ((rc = static_scratch(cfg->device, &base)))
+		return rc;
+	Counters *ctr = (Counters *)base;
+	Fallback *fb = (Fallback *)(base + SCRATCH_FB_OFF);
+
+	SCKG(launch_icond(p, data, info, steps, n, st));
+	if (cfg->precision == GRAYHIP_FP64) {
+		sr.n_dump = 1;
+		for (size_t d = 0; d < n_dump; ++d)
+			SCKG(launch_evolve_f64(p, sr, data, info, steps, n, st));
+		return GRAYHIP_OK;
+	}
+	/* one pool pass over every ray, then the FP64 re-integration of its
+	 * hand-overs (at most one per ray: the queue holds n entries) */
+	int bpc = 0, n_cu = 0;
+	SCKG(pool_f32_blocks_per_cu(p.n_freq, &bpc));
+	SCKG(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, cfg->device));
+	const size_t need = (n + 255) / 256, full = (size_t)(bpc > 0 ? bpc : 1) * (size_t)n_cu;
+	const unsigned blocks = (unsigned)(need < full ? need : full);
+	const RayRange rr = {0, (int)n, (int)n, (int)n};
+	SCKG(launch_zero_counters(ctr, 1, st));
+	SCKG(launch_pool_f32(p, sr, data, info, steps, n, rr, ctr, fb, blocks, st));
+	SCKG(launch_fallback_f64(p, sr, data, info, steps, n, fb, ctr, (unsigned)n, st));
 	return GRAYHIP_OK;
 }

@@ -128,6 +128,23 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 	}
 }
 
+/* Counters cleared by a kernel rather than a memset, so that the clear is an
+ * ordinary kernel node when a caller captures the launches in a graph. */
+__global__ void k_zero_counters(Counters *ctr, int n)
+{
+	const int i = (int)threadIdx.x;
+	if (i < n)
+		ctr[i] = Counters{0u, 0u, 0ull, 0ull};
+}
+
+hipError_t launch_zero_counters(Counters *ctr, int n, hipStream_t st)
+{
+	if (n <= 0 || n > 256)
+		return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(256), 0, st, ctr, n);
+	return hipGetLastError();
+}
+
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks)
 {
 	*blocks = 0;

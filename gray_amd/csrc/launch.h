@@ -49,6 +49,8 @@ hipError_t launch_covariant(double a, const double *data, double *out, size_t n_
                             hipStream_t st);
 
 hipError_t pool_f32_blocks_per_cu(int n_freq, int *blocks);
+/* Clear n <= 256 Counters with a kernel (no memset node). */
+hipError_t launch_zero_counters(Counters *ctr, int n, hipStream_t st);
 hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data, double *info,
                            int64_t *steps, size_t n_rays, const RayRange &rr, Counters *ctr,
                            Fallback *fb, unsigned n_blocks, hipStream_t st);

@@ -21,7 +21,7 @@ EXPORTS = (
     "grayhip_create", "grayhip_destroy", "grayhip_icond", "grayhip_evolve",
     "grayhip_render", "grayhip_read", "grayhip_read_f64", "grayhip_write",
     "grayhip_read_steps", "grayhip_stats", "grayhip_pass_steps", "grayhip_kernel_time",
-    "grayhip_n_data", "grayhip_read_bl", "grayhip_read_covariant",
+    "grayhip_n_data", "grayhip_read_bl", "grayhip_read_covariant", "grayhip_render_static",
     "grayhip_last_error", "grayhip_global_error", "grayhip_device_count",
     "grayhip_spacetime_create", "grayhip_spacetime_destroy", "grayhip_spacetime_upload",
     "grayhip_spacetime_shift", "grayhip_spacetime_set_frozen", "grayhip_spacetime_bytes",
@@ -93,6 +93,8 @@ def lib():
     L.grayhip_n_data.argtypes = [vp]
     L.grayhip_read_bl.argtypes = [vp, C.c_double, vp]
     L.grayhip_read_covariant.argtypes = [vp, vp]
+    L.grayhip_render_static.argtypes = [C.POINTER(Config)] + [C.c_double] * 6 + \
+        [C.c_size_t, C.c_size_t, vp, vp, vp, vp]
     L.grayhip_last_error.argtypes = [vp]
     L.grayhip_last_error.restype = C.c_char_p
     L.grayhip_global_error.argtypes = []

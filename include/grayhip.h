@@ -277,6 +277,27 @@ int  grayhip_server_result(grayhip_server *srv, int64_t id, double *data, double
 int  grayhip_server_stats(grayhip_server *srv, int64_t steps[3], int64_t *epochs);
 const char *grayhip_server_error(const grayhip_server *srv);
 
+/* ---- a render that allocates nothing ----
+ *
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  icond + render of grayhip_icond / grayhip_render for
+ * an analytic-Kerr config (spacetime must be NULL) into caller-owned device
+ * buffers: data SoA [8 + 2 n_freq][W*H] doubles (state s of ray g at
+ * data[s*W*H + g]), info [W*H] doubles of g(u,u), steps [W*H] RK4 steps per
+ * ray.  The FP32 form's scratch (pass counters and the FP64 re-integration
+ * queue, 16 B per ray) is a fixed device array inside the library, one per
+ * GPU, held until the process ends: W*H may not exceed
+ * GRAYHIP_STATIC_MAX_RAYS (GRAYHIP_EINVAL otherwise).  Work is queued on
+ * `stream` (a hipStream_t; NULL is the null stream) and the call returns
+ * without waiting; results equal grayhip_render's bit for bit.  Calls that
+ * overlap on one GPU share that scratch and race: serialise them. */
+#define GRAYHIP_STATIC_MAX_RAYS (1 << 20)
+int  grayhip_render_static(const grayhip_config *cfg, double w_img, double h_img,
+                           double r_obs, double i_obs, double j_obs, double dt,
+                           size_t n_sub, size_t n_dump, double *data, double *info,
+                           int64_t *steps, void *stream);
+
 int  grayhip_n_data(const grayhip_ctx *ctx);
 const char *grayhip_last_error(const grayhip_ctx *ctx);
 const char *grayhip_global_error(void);
