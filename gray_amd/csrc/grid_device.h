@@ -145,6 +145,46 @@ __device__ __forceinline__ void make_stencil(const GridView<S> &G, T qt, T qx, T
 	}
 }
 
+/* Tile (8x8 rays) of wave wv of block b, in a grid of tw x th tiles.
+ * Row-major order (xcd_patches = false): tile waves b + wv.  XCD order:
+ * blocks reach the 8 XCDs round robin (block b on XCD b mod 8), each XCD
+ * with its own L2; tiles are grouped in patches of TILE_PATCH x TILE_PATCH
+ * and patch c is dealt to XCD c mod 8, so that rays which share grid cells
+ * share one L2, while the patches an XCD holds lie all over the image (an
+ * even share of the costly rays near the hole). */
+constexpr unsigned TILE_PATCH = 4;
+
+__host__ __device__ inline unsigned tile_blocks(unsigned tw, unsigned th, unsigned waves,
+                                                bool xcd_patches)
+{
+	if (!xcd_patches)
+		return (tw * th + waves - 1) / waves;
+	const unsigned np = ((tw + TILE_PATCH - 1) / TILE_PATCH) * ((th + TILE_PATCH - 1) / TILE_PATCH);
+	return ((np + 7) / 8) * 8 * (TILE_PATCH * TILE_PATCH / waves);
+}
+
+__host__ __device__ inline bool tile_of(unsigned b, unsigned wv, unsigned waves, unsigned tw,
+                                        unsigned th, bool xcd_patches, unsigned &tx, unsigned &ty)
+{
+	if (!xcd_patches) {
+		const unsigned t = b * waves + wv;
+		tx = t % tw;
+		ty = t / tw;
+		return t < tw * th;
+	}
+	const unsigned ptw = (tw + TILE_PATCH - 1) / TILE_PATCH;
+	const unsigned np = ptw * ((th + TILE_PATCH - 1) / TILE_PATCH);
+	const unsigned bpp = TILE_PATCH * TILE_PATCH / waves;   /* blocks per patch */
+	const unsigned k = b / 8u;
+	const unsigned c = (k / bpp) * 8u + b % 8u;
+	if (c >= np)
+		return false;
+	const unsigned t = (k % bpp) * waves + wv;
+	tx = (c % ptw) * TILE_PATCH + t % TILE_PATCH;
+	ty = (c / ptw) * TILE_PATCH + t / TILE_PATCH;
+	return tx < tw && ty < th;
+}
+
 /* LDS bricks: a wave's copy of 4x4x4 grid points (3x3x3 cells) from
  * origin o; a cell c lies in it when 0 <= c - o <= 2 on every axis. */
 __device__ __forceinline__ bool in_brick(const int c[3], const int o[3])

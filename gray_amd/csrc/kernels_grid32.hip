@@ -10,6 +10,7 @@
  * read and 40 FMAs, and no per-field interpolation state is kept.
  */
 #include <climits>
+#include <cstdlib>
 
 #include "grid_f32.h"
 #include "launch.h"
@@ -236,11 +237,12 @@ __device__ __forceinline__ void grid_rk4_brick(const GridView<S> &G, const RtThe
 	}
 }
 
-/* One launch = n_dump intervals of every ray; wave w of block b holds tile
- * 4b + w (8x8 rays).  All resident waves start together and advance at
- * similar rates, so the rays in flight sweep the grid as one front and the
- * cells they read stay in L2.  Blocks reach the 8 XCDs round robin, which
- * gives every XCD an even share of the costly tiles near the hole. */
+/* One launch = n_dump intervals of every ray; each wave holds one tile of
+ * 8x8 rays, tiles dealt to the XCDs in 4x4-tile patches (tile_of,
+ * grid_device.h), so neighbouring rays share an L2 while every XCD gets
+ * patches from all over the image.  All resident waves start together
+ * and advance at similar rates, so the rays in flight sweep the grid as one
+ * front and the cells they read stay in L2. */
 template <int NF, class S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF == 0 ? 2 : 1)))
 void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__restrict__ data,
@@ -262,15 +264,15 @@ void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__rest
 	wave_sync();
 
 	const unsigned tw = (unsigned)(W + 7) / 8u, th = (unsigned)(H + 7) / 8u;
-	const unsigned tile = blockIdx.x * GP_WAVES + (unsigned)wv;
-	if (tile >= tw * th)
+	unsigned tx, ty;
+	if (!tile_of(blockIdx.x, (unsigned)wv, GP_WAVES, tw, th, (bricks & 2) != 0, tx, ty))
 		return;
-	const unsigned x = (tile % tw) * 8u + (lane & 7u), y = (tile / tw) * 8u + (lane >> 3);
+	const unsigned x = tx * 8u + (lane & 7u), y = ty * 8u + (lane >> 3);
 	if (x >= (unsigned)W || y >= (unsigned)H)
 		return;
 	const size_t g = (size_t)y * (size_t)W + x;
 
-	const bool use_lds = bricks && G.n[0] >= GB && G.n[1] >= GB && G.n[2] >= GB;
+	const bool use_lds = (bricks & 1) && G.n[0] >= GB && G.n[1] >= GB && G.n[2] >= GB;
 	const RtThermalF rt = rt_thermal_f32(rt64);
 	const double inv_scale = NF > 0 ? 1.0 / rt64.scale : 1.0;
 	float hi[ND], lo[ND];
@@ -337,10 +339,13 @@ hipError_t launch_grid_tiles_f32(const GridDesc &gd, const RtThermal &rt, const 
 	if (W <= 0 || H <= 0 || (size_t)W * (size_t)H != n_rays)
 		return hipErrorInvalidValue;
 	const size_t tiles = (((size_t)W + 7) / 8) * (((size_t)H + 7) / 8);
-	const size_t nb = (tiles + GP_WAVES - 1) / GP_WAVES;
-	if (nb > (size_t)INT_MAX)
+	if (tiles > (size_t)INT_MAX / 2)
 		return hipErrorInvalidValue;
-	const int b = bricks ? 1 : 0;
+	/* tiles in XCD patches (grid_device.h: tile_of); measured on 512^2 rays
+	 * through the 128^3 grid: FP32 form 300 -> 287 ms, FP64 800 -> 737 ms */
+	const bool xcd = true;
+	const size_t nb = tile_blocks((unsigned)(W + 7) / 8u, (unsigned)(H + 7) / 8u, GP_WAVES, xcd);
+	const int b = (bricks ? 1 : 0) | (xcd ? 2 : 0);
 	if (gd.dtype == 4) {
 #define L(NF) grid_tiles_launch<NF, float>(gd, rt, sr, data, steps, n_rays, W, H, b, n_steps, \
 	                                   (unsigned)nb, st)
