@@ -1,157 +1,469 @@
 /*
- * grayhip_server.hip -- the serving pool (include/grayhip.h,
- * grayhip_server_*): request queue, admission by an HBM budget, slot
- * refill, and one bounded epoch per step over three slot lists (FP32 Kerr,
- * FP64 Kerr re-integration, FP32 grid) on three streams, so the FP64 work
- * runs beside the FP32 work instead of after it.
+ * grayhip_server.hip -- the serving pool's C ABI (include/grayhip.h,
+ * grayhip_server_*) and its HIP back-end.
+ *
+ * The host scheduler (srv_sched.h) decides admissions and each epoch's
+ * queues; the back-end here owns the device memory (request table, Kerr
+ * classes, queue segments, the two resume, FP64 and tile lists that swap
+ * roles every epoch, the output arena) and runs an epoch as three
+ * persistent kernels on three streams side by side: FP32 Kerr pool, FP64
+ * re-integration, gridded tiles (kernels_serve.hip).
  */
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <deque>
-#include <map>
-#include <mutex>
-#include <vector>
+#include <new>
 
 #include "launch_serve.h"
 #include "spacetime.h"
+#include "srv_sched.h"
 
 using namespace gray;
 
 namespace {
 
-struct HReq {
-	int64_t id;
-	int state;
-	int tab;                 /* index in the device request table */
-	int kind, nd;
-	size_t n_rays, next, remaining;
-	size_t off, bytes;       /* output region in the arena */
-	SrvRequest dev;
-};
+constexpr int SRV_FB_CAP = 1 << 19;     /* entries of each FP64 list          */
+constexpr int SRV_MAX_SEGS = 1 << 15;   /* fresh segments per kind per epoch  */
+constexpr double SRV_EPOCH_MS = 8.0;    /* target epoch duration              */
 
-/* First-fit allocator of arena offsets with coalescing. */
-struct Arena {
-	std::map<size_t, size_t> free_; /* offset -> size */
-	void init(size_t size) { free_.clear(); free_[0] = size; }
-	bool alloc(size_t bytes, size_t *off)
+/* Prograde ISCO (Bardeen, Press & Teukolsky 1972), M = 1. */
+double r_isco_host(double a)
+{
+	const double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+	const double z2 = sqrt(3.0 * a * a + z1 * z1);
+	return 3.0 + z2 - sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
+class HipDevice : public srv::Device {
+public:
+	grayhip_server_config cfg;
+	srv::Caps caps;
+	hipStream_t s[3] = {nullptr, nullptr, nullptr};
+	hipEvent_t ev[7] = {};
+	SrvRequest *d_reqs = nullptr;
+	unsigned *d_remaining = nullptr;
+	SrvClass *d_cls = nullptr;
+	int *d_kseg = nullptr, *d_gseg = nullptr;        /* tab, first, start (3 x max) */
+	SrvResume res[2] = {};
+	SrvFb fb[2] = {};
+	SrvTiles tiles[2] = {};
+	SrvCtr *d_ctr = nullptr;
+	int *d_done = nullptr;
+	char *arena = nullptr;
+	/* pinned staging */
+	SrvCtr *h_ctr = nullptr;
+	int *h_done = nullptr;
+	SrvClass *h_cls = nullptr;
+	int *h_kseg = nullptr, *h_gseg = nullptr;
+	/* messages per calling thread: an epoch and a result copy may fail on
+	 * different threads at once */
+	static char *err_buf()
 	{
-		for (auto it = free_.begin(); it != free_.end(); ++it)
-			if (it->second >= bytes) {
-				*off = it->first;
-				size_t rest = it->second - bytes, noff = it->first + bytes;
-				free_.erase(it);
-				if (rest)
-					free_[noff] = rest;
-				return true;
-			}
-		return false;
+		static thread_local char err[256];
+		return err;
 	}
-	void release(size_t off, size_t bytes)
+
+	~HipDevice() override { release(); }
+
+	int fail(int code, const char *fmt, ...)
 	{
-		auto it = free_.emplace(off, bytes).first;
-		auto nx = std::next(it);
-		if (nx != free_.end() && it->first + it->second == nx->first) {
-			it->second += nx->second;
-			free_.erase(nx);
+		va_list ap;
+		va_start(ap, fmt);
+		vsnprintf(err_buf(), 256, fmt, ap);
+		va_end(ap);
+		return code;
+	}
+	const char *error() const override { return err_buf(); }
+
+	int init(const grayhip_server_config &c)
+	{
+		cfg = c;
+		hipError_t e = hipSetDevice(c.device);
+		int n_cu = 0, kb = 0, gb = 0;
+		if (e == hipSuccess)
+			e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device);
+		if (e == hipSuccess)
+			e = srv_kerr_blocks_per_cu(&kb);
+		if (e == hipSuccess)
+			e = srv_grid_blocks_per_cu(&gb);
+		if (e != hipSuccess)
+			return fail(GRAYHIP_EDEVICE, "device query: %s", hipGetErrorString(e));
+		/* `slots` bounds the rays in flight of each launch */
+		const int want = (int)std::min<size_t>((c.slots + SRV_BLOCK - 1) / SRV_BLOCK, 1u << 20);
+		caps.kerr_blocks = std::max(1, std::min(std::max(kb, 1) * n_cu, want));
+		caps.grid_blocks = std::max(1, std::min(std::max(gb, 1) * n_cu, want));
+		caps.fb_cap = SRV_FB_CAP;
+		caps.max_segs = SRV_MAX_SEGS;
+		caps.max_requests = c.max_requests;
+		caps.arena = c.hbm_budget;
+		const size_t R = (size_t)c.max_requests;
+		const int kcap = (caps.kerr_blocks + SRV_MAX_CLASSES) * SRV_BLOCK;
+		const int gcap = caps.grid_blocks * SRV_GWAVES + SRV_GWAVES;
+		struct { void **p; size_t b; } al[] = {
+			{(void **)&d_reqs, R * sizeof(SrvRequest)},
+			{(void **)&d_remaining, R * sizeof(unsigned)},
+			{(void **)&d_cls, SRV_MAX_CLASSES * sizeof(SrvClass)},
+			{(void **)&d_kseg, 3 * (size_t)SRV_MAX_SEGS * sizeof(int)},
+			{(void **)&d_gseg, 3 * (size_t)SRV_MAX_SEGS * sizeof(int)},
+			{(void **)&res[0].meta, (size_t)kcap * sizeof(int4)},
+			{(void **)&res[1].meta, (size_t)kcap * sizeof(int4)},
+			{(void **)&res[0].steps, (size_t)kcap * sizeof(int64_t)},
+			{(void **)&res[1].steps, (size_t)kcap * sizeof(int64_t)},
+			{(void **)&res[0].st, (size_t)kcap * 2 * SRV_NK * sizeof(float)},
+			{(void **)&res[1].st, (size_t)kcap * 2 * SRV_NK * sizeof(float)},
+			{(void **)&fb[0].meta, (size_t)SRV_FB_CAP * sizeof(int4)},
+			{(void **)&fb[1].meta, (size_t)SRV_FB_CAP * sizeof(int4)},
+			{(void **)&fb[0].steps, (size_t)SRV_FB_CAP * sizeof(int64_t)},
+			{(void **)&fb[1].steps, (size_t)SRV_FB_CAP * sizeof(int64_t)},
+			{(void **)&fb[0].x, (size_t)SRV_FB_CAP * SRV_NC * sizeof(double)},
+			{(void **)&fb[1].x, (size_t)SRV_FB_CAP * SRV_NC * sizeof(double)},
+			{(void **)&tiles[0].hdr, (size_t)gcap * sizeof(int2)},
+			{(void **)&tiles[1].hdr, (size_t)gcap * sizeof(int2)},
+			{(void **)&tiles[0].brick, (size_t)gcap * sizeof(int4)},
+			{(void **)&tiles[1].brick, (size_t)gcap * sizeof(int4)},
+			{(void **)&tiles[0].meta, (size_t)gcap * 64 * sizeof(int4)},
+			{(void **)&tiles[1].meta, (size_t)gcap * 64 * sizeof(int4)},
+			{(void **)&tiles[0].steps, (size_t)gcap * 64 * sizeof(int64_t)},
+			{(void **)&tiles[1].steps, (size_t)gcap * 64 * sizeof(int64_t)},
+			{(void **)&tiles[0].st, (size_t)gcap * 64 * 2 * SRV_NG * sizeof(float)},
+			{(void **)&tiles[1].st, (size_t)gcap * 64 * 2 * SRV_NG * sizeof(float)},
+			{(void **)&d_ctr, sizeof(SrvCtr)},
+			{(void **)&d_done, R * sizeof(int)},
+			{(void **)&arena, c.hbm_budget},
+		};
+		for (auto &a : al)
+			if (e == hipSuccess)
+				e = hipMalloc(a.p, a.b);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_ctr, sizeof(SrvCtr), 0);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_done, R * sizeof(int), 0);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_cls, SRV_MAX_CLASSES * sizeof(SrvClass), 0);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_kseg, 3 * (size_t)SRV_MAX_SEGS * sizeof(int), 0);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_gseg, 3 * (size_t)SRV_MAX_SEGS * sizeof(int), 0);
+		for (int i = 0; i < 3 && e == hipSuccess; ++i)
+			e = hipStreamCreateWithFlags(&s[i], hipStreamNonBlocking);
+		for (int i = 0; i < 7 && e == hipSuccess; ++i)
+			e = hipEventCreate(&ev[i]);
+		if (e != hipSuccess)
+			return fail(GRAYHIP_ENOMEM, "server allocation: %s", hipGetErrorString(e));
+		for (int k = 0; k < 2; ++k) {
+			res[k].cap = kcap;
+			fb[k].cap = SRV_FB_CAP;
+			tiles[k].cap = gcap;
 		}
-		if (it != free_.begin()) {
-			auto pv = std::prev(it);
-			if (pv->first + pv->second == it->first) {
-				pv->second += it->second;
-				free_.erase(it);
+		return GRAYHIP_OK;
+	}
+
+	void release()
+	{
+		(void)hipSetDevice(cfg.device);
+		for (int i = 0; i < 3; ++i)
+			if (s[i])
+				(void)hipStreamDestroy(s[i]);
+		for (int i = 0; i < 7; ++i)
+			if (ev[i])
+				(void)hipEventDestroy(ev[i]);
+		void *dev[] = {d_reqs, d_remaining, d_cls, d_kseg, d_gseg, d_ctr, d_done, arena};
+		for (void *p : dev)
+			(void)hipFree(p);
+		for (int k = 0; k < 2; ++k) {
+			(void)hipFree(res[k].meta);
+			(void)hipFree(res[k].steps);
+			(void)hipFree(res[k].st);
+			(void)hipFree(fb[k].meta);
+			(void)hipFree(fb[k].steps);
+			(void)hipFree(fb[k].x);
+			(void)hipFree(tiles[k].hdr);
+			(void)hipFree(tiles[k].brick);
+			(void)hipFree(tiles[k].meta);
+			(void)hipFree(tiles[k].steps);
+			(void)hipFree(tiles[k].st);
+		}
+		void *host[] = {h_ctr, h_done, h_cls, h_kseg, h_gseg};
+		for (void *p : host)
+			(void)hipHostFree(p);
+	}
+
+	int admit(int tab, const srv::Req &r) override
+	{
+		const grayhip_request *q = &r.q;
+		SrvRequest D;
+		memset(&D, 0, sizeof(D));
+		RayParams &p = D.p;
+		const int nf = q->n_freq;
+		p.a = q->spacetime ? 0.0 : q->a_spin;
+		p.r_h = 1.0 + sqrt(1.0 - p.a * p.a);
+		p.escape = q->escape;
+		p.eps_stop = q->eps_stop > 0.0 ? q->eps_stop
+		             : (q->escape == GRAYHIP_ESC_OUTWARD ? 0.01 * p.r_h : 0.01);
+		p.r_esc = q->r_esc;
+		p.r_isco = r_isco_host(p.a);
+		p.n_freq = nf;
+		p.fluid = nf > 0 ? q->fluid : GRAYHIP_FLUID_NONE;
+		for (int i = 0; i < GRAYHIP_MAX_FREQ; ++i)
+			p.nu[i] = nf > 0 ? q->nu[i < nf && i < SRV_NF ? i : 0] : 0.0;
+		const double d2r = M_PI / 180.0;
+		p.w_img = q->w_img;
+		p.h_img = q->h_img;
+		p.r_obs = q->r_obs;
+		p.si = sin(d2r * q->i_obs);
+		p.ci = cos(d2r * q->i_obs);
+		p.sj = sin(d2r * q->j_obs);
+		p.cj = cos(d2r * q->j_obs);
+		p.w_rays = (int)q->w_rays;
+		p.h_rays = (int)q->h_rays;
+		StepRule &sr = D.sr;
+		sr.dt = q->dt;
+		sr.dts = q->dt / (double)q->n_sub;
+		sr.n = (int)((INT32_MAX / (long)q->n_sub) * (long)q->n_sub);
+		sr.n_dump = (int)q->n_dump;
+		if (q->spacetime) {
+			D.gv = spacetime_view(q->spacetime, q->t_init, 0, q->escape == GRAYHIP_ESC_REF)
+			           .view<float>();
+			rt_thermal_init(&D.rt, 4.3e6, nf, q->nu);
+		}
+		D.kind = r.kind;
+		D.nd = r.nd;
+		D.cls = r.cls;
+		D.tw = (int)((q->w_rays + 7) / 8);
+		double *base = (double *)(arena + r.off);
+		D.out_data = base;
+		D.out_info = base + r.n_rays * (size_t)r.nd;
+		D.out_steps = (int64_t *)(D.out_info + r.n_rays);
+		const unsigned rem = (unsigned)r.n_rays;
+		hipError_t e = hipSetDevice(cfg.device);
+		if (e == hipSuccess)
+			e = hipMemcpy(d_reqs + tab, &D, sizeof(D), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(d_remaining + tab, &rem, sizeof(rem), hipMemcpyHostToDevice);
+		if (e != hipSuccess)
+			return fail(GRAYHIP_EDEVICE, "admit: %s", hipGetErrorString(e));
+		return GRAYHIP_OK;
+	}
+
+	/* Segments of one kind into the pinned staging and the device: tab,
+	 * first, and each segment's first queue item within its class. */
+	static void pack_segs(const std::vector<srv::Seg> &segs, const srv::Plan &p, bool kerr, int *h)
+	{
+		const size_t n = segs.size();
+		int *tab = h, *first = h + SRV_MAX_SEGS, *start = h + 2 * SRV_MAX_SEGS;
+		for (size_t j = 0; j < n; ++j) {
+			tab[j] = segs[j].tab;
+			first[j] = segs[j].first;
+		}
+		if (kerr) {
+			for (int k = 0; k < p.n_cls; ++k) {
+				unsigned acc = 0;
+				for (int j = p.cls[k].seg0; j < p.cls[k].seg1; ++j) {
+					start[j] = (int)acc;
+					acc += segs[(size_t)j].count;
+				}
+			}
+		} else {
+			unsigned acc = 0;
+			for (size_t j = 0; j < n; ++j) {
+				start[j] = (int)acc;
+				acc += segs[j].count;
 			}
 		}
 	}
+
+	hipError_t put_segs(const int *h, int *d, size_t n, hipStream_t st)
+	{
+		hipError_t e = hipSuccess;
+		for (int k = 0; k < 3 && e == hipSuccess && n > 0; ++k)
+			e = hipMemcpyAsync(d + (size_t)k * SRV_MAX_SEGS, h + (size_t)k * SRV_MAX_SEGS,
+			                   n * sizeof(int), hipMemcpyHostToDevice, st);
+		return e;
+	}
+
+#define DCK(expr)                                                                         \
+	do {                                                                              \
+		hipError_t e_ = (expr);                                                   \
+		if (e_ != hipSuccess)                                                     \
+			return fail(GRAYHIP_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+	} while (0)
+
+	int epoch(const srv::Plan &p, srv::Result &out) override
+	{
+		DCK(hipSetDevice(cfg.device));
+		if (p.kseg.size() > (size_t)SRV_MAX_SEGS || p.gseg.size() > (size_t)SRV_MAX_SEGS)
+			return fail(GRAYHIP_EDEVICE, "too many queue segments");
+		/* the lists the kernels index: Kerr lanes save into [block0 256,
+		 * block1 256) of a resume list, grid waves one record each */
+		if (p.kerr_blocks > caps.kerr_blocks + SRV_MAX_CLASSES || p.grid_blocks > caps.grid_blocks ||
+		    p.n_fb > SRV_FB_CAP || p.g_resume > tiles[0].cap || p.n_cls > SRV_MAX_CLASSES)
+			return fail(GRAYHIP_EDEVICE, "epoch plan beyond the pool's lists");
+		for (int k = 0; k < p.n_cls; ++k)
+			if (p.cls[k].block1 > p.kerr_blocks ||
+			    p.cls[k].res_off + p.cls[k].n_resume > res[0].cap)
+				return fail(GRAYHIP_EDEVICE, "class plan beyond the resume lists");
+		const int in = p.flip, o = p.flip ^ 1;
+		for (int k = 0; k < p.n_cls; ++k) {
+			const srv::ClassPlan &cp = p.cls[k];
+			SrvClass &C = h_cls[k];
+			memset(&C, 0, sizeof(C));
+			C.sr.dt = cp.key.dt;
+			C.sr.dts = cp.key.dt / (double)cp.key.n_sub;
+			C.sr.n = (int)((INT32_MAX / (long)cp.key.n_sub) * (long)cp.key.n_sub);
+			C.sr.n_dump = (int)cp.key.n_dump;
+			for (int i = 0; i < SRV_NF; ++i)
+				C.nu[i] = cp.key.nu[i];
+			C.r_esc = cp.key.r_esc;
+			C.n_freq = cp.key.n_freq;
+			C.fluid = cp.key.n_freq > 0 ? cp.key.fluid : GRAYHIP_FLUID_NONE;
+			C.escape = cp.key.escape;
+			C.block0 = cp.block0;
+			C.block1 = cp.block1;
+			C.res_off = cp.res_off;
+			C.n_resume = cp.n_resume;
+			C.seg0 = cp.seg0;
+			C.seg1 = cp.seg1;
+			C.items = cp.items;
+		}
+		pack_segs(p.kseg, p, true, h_kseg);
+		pack_segs(p.gseg, p, false, h_gseg);
+		hipStream_t A = s[0];
+		if (p.n_cls > 0)
+			DCK(hipMemcpyAsync(d_cls, h_cls, (size_t)p.n_cls * sizeof(SrvClass),
+			                   hipMemcpyHostToDevice, A));
+		DCK(put_segs(h_kseg, d_kseg, p.kseg.size(), A));
+		DCK(put_segs(h_gseg, d_gseg, p.gseg.size(), A));
+		DCK(launch_srv_clear(d_ctr, A));
+		DCK(hipEventRecord(ev[0], A));
+		DCK(hipStreamWaitEvent(s[1], ev[0], 0));
+		DCK(hipStreamWaitEvent(s[2], ev[0], 0));
+
+		SrvKerrArgs K;
+		K.reqs = d_reqs;
+		K.cls = d_cls;
+		K.n_cls = p.n_cls;
+		K.seg = SrvSegs{d_kseg, d_kseg + SRV_MAX_SEGS, (const unsigned *)(d_kseg + 2 * SRV_MAX_SEGS)};
+		K.in = res[in];
+		K.out = res[o];
+		K.fb = fb[o];
+		K.ctr = d_ctr;
+		K.remaining = d_remaining;
+		K.done = d_done;
+		K.K = p.K_kerr;
+		SrvFbArgs F;
+		F.reqs = d_reqs;
+		F.in = fb[in];
+		F.n_in = p.n_fb;
+		F.out = fb[o];
+		F.ctr = d_ctr;
+		F.remaining = d_remaining;
+		F.done = d_done;
+		F.K = p.K_fb;
+		SrvGridArgs Gd;
+		Gd.reqs = d_reqs;
+		Gd.seg = SrvSegs{d_gseg, d_gseg + SRV_MAX_SEGS, (const unsigned *)(d_gseg + 2 * SRV_MAX_SEGS)};
+		Gd.seg0 = 0;
+		Gd.seg1 = (int)p.gseg.size();
+		Gd.n_resume = p.g_resume;
+		Gd.items = p.g_items;
+		Gd.in = tiles[in];
+		Gd.out = tiles[o];
+		Gd.ctr = d_ctr;
+		Gd.remaining = d_remaining;
+		Gd.done = d_done;
+		Gd.K = p.K_grid;
+
+		const bool run[3] = {p.n_cls > 0 && p.kerr_blocks > 0, p.n_fb > 0, p.g_items > 0};
+		for (int k = 0; k < 3; ++k) {
+			if (!run[k])
+				continue;
+			DCK(hipEventRecord(ev[1 + 2 * k], s[k]));
+			if (k == 0)
+				DCK(launch_srv_kerr(K, p.kerr_blocks, s[0]));
+			else if (k == 1)
+				DCK(launch_srv_fb64(F, s[1]));
+			else
+				DCK(launch_srv_grid(Gd, p.grid_blocks, s[2]));
+			DCK(hipEventRecord(ev[2 + 2 * k], s[k]));
+		}
+		for (int k = 0; k < 3; ++k)
+			DCK(hipStreamSynchronize(s[k]));
+		DCK(hipMemcpy(h_ctr, d_ctr, sizeof(SrvCtr), hipMemcpyDeviceToHost));
+		if (h_ctr->n_done > (unsigned)cfg.max_requests)
+			return fail(GRAYHIP_EDEVICE, "done list overflowed");
+		if (h_ctr->n_done)
+			DCK(hipMemcpy(h_done, d_done, h_ctr->n_done * sizeof(int), hipMemcpyDeviceToHost));
+		for (int k = 0; k < SRV_MAX_CLASSES; ++k) {
+			out.claimed[k] = h_ctr->claimed[k];
+			out.saved[k] = h_ctr->saved[k];
+		}
+		out.fb_out = h_ctr->fb_out;
+		out.g_claimed = h_ctr->g_claimed;
+		out.g_saved = h_ctr->g_saved;
+		out.done.assign(h_done, h_done + h_ctr->n_done);
+		for (int k = 0; k < 3; ++k) {
+			out.steps[k] = h_ctr->steps[k];
+			out.ms[k] = -1.0;
+		}
+		/* kernel times in scheduler order: Kerr, FP64, grid */
+		for (int k = 0; k < 3; ++k) {
+			float ms = 0.f;
+			if (run[k] && hipEventElapsedTime(&ms, ev[1 + 2 * k], ev[2 + 2 * k]) == hipSuccess)
+				out.ms[k] = (double)ms;
+		}
+		return GRAYHIP_OK;
+	}
+
+	int fetch(const srv::Req &r, double *data, double *info, int64_t *steps) override
+	{
+		DCK(hipSetDevice(cfg.device));
+		const double *base = (const double *)(arena + r.off);
+		if (data)
+			DCK(hipMemcpy(data, base, r.n_rays * (size_t)r.nd * 8, hipMemcpyDeviceToHost));
+		if (info)
+			DCK(hipMemcpy(info, base + r.n_rays * (size_t)r.nd, r.n_rays * 8,
+			              hipMemcpyDeviceToHost));
+		if (steps)
+			DCK(hipMemcpy(steps, base + r.n_rays * (size_t)(r.nd + 1), r.n_rays * 8,
+			              hipMemcpyDeviceToHost));
+		return GRAYHIP_OK;
+	}
+#undef DCK
 };
 
 } // namespace
 
 struct grayhip_server {
-	grayhip_server_config cfg;
-	hipStream_t s[3];
-	hipEvent_t ev_ready;
-	SrvRequest *d_reqs;
-	SrvSlots S;
-	int *d_list[3];
-	int *d_assign;            /* 3 x slots: slot, table index, ray */
-	SrvLists L;
-	char *arena;
-	Arena alloc;
-	/* pinned host mirrors */
-	SrvCounters *h_ctr;
-	int *h_fin, *h_to64, *h_assign;
-	/* host bookkeeping */
-	std::map<int64_t, HReq> reqs;
-	std::deque<int64_t> pending, filling, done;
-	std::vector<int> free_tab, free_slots;
-	std::vector<int64_t> slot_req;   /* request id per slot */
-	std::vector<int> slot_list, slot_pos;
-	std::vector<int> list[3];
-	int64_t next_id;
-	int64_t epochs;
-	unsigned long long steps[3];
-	/* Guards the host bookkeeping (requests, queues, arena, slot lists) so
-	 * submit / next_done / state / result / stats may run on other threads
-	 * while one thread steps; not held while an epoch runs on the GPU. */
-	std::mutex mu;
-	char err[256];
+	HipDevice dev;
+	srv::Sched *sched = nullptr;
 };
 
-static int vfail(grayhip_server *v, int code, const char *fmt, ...)
+/* The message of the calling thread's last failed server call. */
+static thread_local char t_err[256];
+
+static int vfail(grayhip_server *, int code, const char *fmt, ...)
 {
-	static thread_local char g[256];
 	va_list ap;
 	va_start(ap, fmt);
-	vsnprintf(v ? v->err : g, 256, fmt, ap);
+	vsnprintf(t_err, sizeof(t_err), fmt, ap);
 	va_end(ap);
 	return code;
 }
 
-#define SCK(v, expr)                                                                   \
-	do {                                                                           \
-		hipError_t e_ = (expr);                                                \
-		if (e_ != hipSuccess)                                                  \
-			return vfail(v, GRAYHIP_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
-	} while (0)
-
-extern "C" const char *grayhip_server_error(const grayhip_server *v)
+extern "C" const char *grayhip_server_error(const grayhip_server *)
 {
-	return v ? v->err : "server: null handle or creation failed";
+	return t_err;
 }
 
 extern "C" void grayhip_server_destroy(grayhip_server *v)
 {
 	if (!v)
 		return;
-	(void)hipSetDevice(v->cfg.device);
-	for (int i = 0; i < 3; ++i) {
-		if (v->s[i])
-			(void)hipStreamDestroy(v->s[i]);
-		(void)hipFree(v->d_list[i]);
-	}
-	if (v->ev_ready)
-		(void)hipEventDestroy(v->ev_ready);
-	(void)hipFree(v->d_reqs);
-	(void)hipFree(v->S.req);
-	(void)hipFree(v->S.ray);
-	(void)hipFree(v->S.d);
-	(void)hipFree(v->S.h);
-	(void)hipFree(v->S.steps);
-	(void)hipFree(v->S.st);
-	(void)hipFree(v->S.st64);
-	(void)hipFree(v->d_assign);
-	(void)hipFree(v->L.fin);
-	(void)hipFree(v->L.to64);
-	(void)hipFree(v->L.ctr);
-	(void)hipFree(v->arena);
-	(void)hipHostFree(v->h_ctr);
-	(void)hipHostFree(v->h_fin);
-	(void)hipHostFree(v->h_to64);
-	(void)hipHostFree(v->h_assign);
+	delete v->sched;
 	delete v;
 }
 
@@ -161,75 +473,32 @@ extern "C" int grayhip_server_create(const grayhip_server_config *cfg, grayhip_s
 		return vfail(nullptr, GRAYHIP_EINVAL, "null argument");
 	*out = nullptr;
 	if (cfg->slots == 0 || cfg->slots > (1u << 28) || cfg->epoch_steps <= 0 ||
-	    cfg->max_requests <= 0 || cfg->hbm_budget == 0)
+	    cfg->max_requests <= 0 || cfg->max_requests > (1 << 20) || cfg->hbm_budget == 0)
 		return vfail(nullptr, GRAYHIP_EINVAL, "bad server configuration");
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
 		return vfail(nullptr, GRAYHIP_ENODEV, "no HIP device visible");
+	if (cfg->device < 0 || cfg->device >= ndev)
+		return vfail(nullptr, GRAYHIP_EINVAL, "device %d out of range", cfg->device);
 	grayhip_server *v = new (std::nothrow) grayhip_server();
 	if (!v)
 		return vfail(nullptr, GRAYHIP_ENOMEM, "host allocation failed");
-	v->cfg = *cfg;
-	const size_t S = cfg->slots, R = (size_t)cfg->max_requests;
-	hipError_t e = hipSetDevice(cfg->device);
-	struct { void **p; size_t b; } al[] = {
-		{(void **)&v->d_reqs, R * sizeof(SrvRequest)},
-		{(void **)&v->S.req, S * sizeof(int)},
-		{(void **)&v->S.ray, S * sizeof(int)},
-		{(void **)&v->S.d, S * sizeof(int)},
-		{(void **)&v->S.h, S * sizeof(int)},
-		{(void **)&v->S.steps, S * sizeof(int64_t)},
-		{(void **)&v->S.st, S * 2 * SRV_NC * sizeof(float)},
-		{(void **)&v->S.st64, S * SRV_NC * sizeof(double)},
-		{(void **)&v->d_list[0], S * sizeof(int)},
-		{(void **)&v->d_list[1], S * sizeof(int)},
-		{(void **)&v->d_list[2], S * sizeof(int)},
-		{(void **)&v->d_assign, 3 * S * sizeof(int)},
-		{(void **)&v->L.fin, S * sizeof(int)},
-		{(void **)&v->L.to64, S * sizeof(int)},
-		{(void **)&v->L.ctr, sizeof(SrvCounters)},
-		{(void **)&v->arena, cfg->hbm_budget},
-	};
-	for (auto &a : al)
-		if (e == hipSuccess)
-			e = hipMalloc(a.p, a.b);
-	if (e == hipSuccess)
-		e = hipHostMalloc((void **)&v->h_ctr, sizeof(SrvCounters), 0);
-	if (e == hipSuccess)
-		e = hipHostMalloc((void **)&v->h_fin, S * sizeof(int), 0);
-	if (e == hipSuccess)
-		e = hipHostMalloc((void **)&v->h_to64, S * sizeof(int), 0);
-	if (e == hipSuccess)
-		e = hipHostMalloc((void **)&v->h_assign, 3 * S * sizeof(int), 0);
-	for (int i = 0; i < 3 && e == hipSuccess; ++i)
-		e = hipStreamCreateWithFlags(&v->s[i], hipStreamNonBlocking);
-	if (e == hipSuccess)
-		e = hipEventCreateWithFlags(&v->ev_ready, hipEventDisableTiming);
-	if (e != hipSuccess) {
-		int rc = vfail(nullptr, GRAYHIP_ENOMEM, "server allocation: %s", hipGetErrorString(e));
-		grayhip_server_destroy(v);
+	int rc = v->dev.init(*cfg);
+	if (rc) {
+		vfail(nullptr, rc, "%s", v->dev.error());
+		delete v;
 		return rc;
 	}
-	v->S.S = (int)S;
-	v->alloc.init(cfg->hbm_budget);
-	for (int i = (int)R - 1; i >= 0; --i)
-		v->free_tab.push_back(i);
-	v->free_slots.reserve(S);
-	for (int i = (int)S - 1; i >= 0; --i)
-		v->free_slots.push_back(i);
-	v->slot_req.assign(S, -1);
-	v->slot_list.assign(S, -1);
-	v->slot_pos.assign(S, -1);
-	v->next_id = 1;
+	srv::Config sc;
+	sc.epoch_steps = cfg->epoch_steps;
+	sc.epoch_ms = SRV_EPOCH_MS;
+	v->sched = new (std::nothrow) srv::Sched(sc, v->dev.caps, &v->dev);
+	if (!v->sched) {
+		delete v;
+		return vfail(nullptr, GRAYHIP_ENOMEM, "host allocation failed");
+	}
 	*out = v;
 	return GRAYHIP_OK;
-}
-
-static double r_isco_host(double a)
-{
-	double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
-	double z2 = sqrt(3.0 * a * a + z1 * z1);
-	return 3.0 + z2 - sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
 }
 
 extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q, int64_t *id)
@@ -243,252 +512,74 @@ extern "C" int grayhip_server_submit(grayhip_server *v, const grayhip_request *q
 	if (q->n_freq < 0 || q->n_freq > SRV_NF)
 		return vfail(v, GRAYHIP_EINVAL, "n_freq must be in [0, %d]", SRV_NF);
 	if (!(fabs(q->a_spin) < 1.0) || q->n_sub == 0 || q->n_sub > (size_t)INT32_MAX ||
-	    q->n_dump == 0 || q->n_dump > (size_t)INT32_MAX || !(q->dt != 0.0))
+	    q->n_dump == 0 || q->n_dump > (size_t)INT32_MAX || !(q->dt != 0.0) ||
+	    !std::isfinite(q->dt))
 		return vfail(v, GRAYHIP_EINVAL, "bad spin or schedule");
-	if (q->n_freq > 0 && q->fluid != GRAYHIP_FLUID_DISK && q->fluid != GRAYHIP_FLUID_UNIFORM)
+	if (q->escape != GRAYHIP_ESC_REF && q->escape != GRAYHIP_ESC_OUTWARD)
+		return vfail(v, GRAYHIP_EINVAL, "unknown escape rule %d", q->escape);
+	if (!(q->r_obs > 0.0))
+		return vfail(v, GRAYHIP_EINVAL, "r_obs must be positive");
+	for (int i = 0; i < q->n_freq; ++i)
+		if (!(q->nu[i] > 0.0))
+			return vfail(v, GRAYHIP_EINVAL, "frequency channel %d must be positive", i);
+	if (q->spacetime) {
+		if (q->spacetime->desc.dtype != 4 || q->spacetime->desc.device != v->dev.cfg.device)
+			return vfail(v, GRAYHIP_EINVAL, "gridded requests need an FP32 spacetime on this "
+			                                "device");
+		if (q->n_freq > 0 && q->fluid != GRAYHIP_FLUID_THERMAL)
+			return vfail(v, GRAYHIP_EINVAL, "transfer on a numerical spacetime uses the "
+			                                "thermal fluid (the reference flow)");
+		if (q->n_freq > 0 && !grayhip_spacetime_has_rho(q->spacetime))
+			return vfail(v, GRAYHIP_ESTATE, "transfer needs the density of every level in "
+			                                "use (grayhip_spacetime_upload_rho)");
+	} else if (q->n_freq > 0 && q->fluid != GRAYHIP_FLUID_DISK &&
+	           q->fluid != GRAYHIP_FLUID_UNIFORM) {
 		return vfail(v, GRAYHIP_EINVAL, "served analytic requests use the disk or uniform fluid");
-	if (q->spacetime && (q->spacetime->desc.dtype != 4 || q->n_freq != 0 ||
-	                     q->spacetime->desc.device != v->cfg.device))
-		return vfail(v, GRAYHIP_EINVAL, "gridded requests need an FP32 spacetime on this "
-		                                "device and no transfer channels");
-	HReq r;
+	}
+	srv::Req r;
 	memset(&r, 0, sizeof(r));
-	r.state = GRAYHIP_REQ_QUEUED;
-	r.tab = -1;
+	r.q = *q;
 	r.kind = q->spacetime ? SRV_GRID : SRV_KERR;
 	r.nd = 8 + 2 * q->n_freq;
 	r.n_rays = q->w_rays * q->h_rays;
-	r.remaining = r.n_rays;
+	r.units = r.kind == SRV_KERR ? r.n_rays : ((q->w_rays + 7) / 8) * ((q->h_rays + 7) / 8);
 	r.bytes = (r.n_rays * (size_t)(r.nd + 2) * 8 + 255) & ~(size_t)255;
-	if (r.bytes > v->cfg.hbm_budget)
-		return vfail(v, GRAYHIP_EINVAL, "request needs %zu bytes, budget is %zu", r.bytes,
-		             v->cfg.hbm_budget);
-	RayParams &p = r.dev.p;
-	p.a = q->spacetime ? 0.0 : q->a_spin;
-	p.r_h = 1.0 + sqrt(1.0 - p.a * p.a);
-	p.escape = q->escape;
-	p.eps_stop = q->eps_stop > 0.0 ? q->eps_stop
-	             : (q->escape == GRAYHIP_ESC_OUTWARD ? 0.01 * p.r_h : 0.01);
-	p.r_esc = q->r_esc;
-	p.r_isco = r_isco_host(p.a);
-	p.n_freq = q->n_freq;
-	p.fluid = q->n_freq > 0 ? q->fluid : GRAYHIP_FLUID_NONE;
-	for (int i = 0; i < q->n_freq; ++i) {
-		if (!(q->nu[i] > 0.0))
-			return vfail(v, GRAYHIP_EINVAL, "frequency channel %d must be positive", i);
-		p.nu[i] = q->nu[i];
+	if (r.kind == SRV_KERR) {
+		srv::ClassKey &k = r.key;
+		k.dt = q->dt;
+		k.n_sub = q->n_sub;
+		k.n_dump = q->n_dump;
+		k.n_freq = q->n_freq;
+		k.fluid = q->n_freq > 0 ? q->fluid : GRAYHIP_FLUID_NONE;
+		k.escape = q->escape;
+		k.r_esc = q->r_esc;
+		for (int i = 0; i < SRV_NF; ++i)
+			k.nu[i] = q->n_freq > 0 ? q->nu[i < q->n_freq ? i : 0] : 0.0;
 	}
-	const double d2r = M_PI / 180.0;
-	p.w_img = q->w_img;
-	p.h_img = q->h_img;
-	p.r_obs = q->r_obs;
-	p.si = sin(d2r * q->i_obs);
-	p.ci = cos(d2r * q->i_obs);
-	p.sj = sin(d2r * q->j_obs);
-	p.cj = cos(d2r * q->j_obs);
-	p.w_rays = (int)q->w_rays;
-	p.h_rays = (int)q->h_rays;
-	StepRule &sr = r.dev.sr;
-	sr.dt = q->dt;
-	sr.dts = q->dt / (double)q->n_sub;
-	sr.n = (int)((INT32_MAX / (long)q->n_sub) * (long)q->n_sub);
-	sr.n_dump = (int)q->n_dump;
-	if (q->spacetime)
-		r.dev.gv = spacetime_view(q->spacetime, q->t_init, 0, q->escape == GRAYHIP_ESC_REF)
-		               .view<float>();
-	r.dev.kind = r.kind;
-	r.dev.nd = r.nd;
-	std::lock_guard<std::mutex> lk(v->mu);
-	r.id = v->next_id++;
-	v->reqs[r.id] = r;
-	v->pending.push_back(r.id);
-	*id = r.id;
-	return GRAYHIP_OK;
-}
-
-static void list_add(grayhip_server *v, int which, int slot)
-{
-	v->slot_list[slot] = which;
-	v->slot_pos[slot] = (int)v->list[which].size();
-	v->list[which].push_back(slot);
-}
-
-static void list_remove(grayhip_server *v, int slot)
-{
-	int w = v->slot_list[slot], p = v->slot_pos[slot];
-	std::vector<int> &L = v->list[w];
-	int last = L.back();
-	L[p] = last;
-	v->slot_pos[last] = p;
-	L.pop_back();
-	v->slot_list[slot] = -1;
-	v->slot_pos[slot] = -1;
-}
-
-static int harvest(grayhip_server *v)
-{
-	SCK(v, hipMemcpy(v->h_ctr, v->L.ctr, sizeof(SrvCounters), hipMemcpyDeviceToHost));
-	const unsigned nf = v->h_ctr->n_fin, n64 = v->h_ctr->n_to64;
-	if (nf > (unsigned)v->S.S || n64 > (unsigned)v->S.S)
-		return vfail(v, GRAYHIP_EDEVICE, "epoch lists overflowed");
-	if (nf)
-		SCK(v, hipMemcpy(v->h_fin, v->L.fin, nf * sizeof(int), hipMemcpyDeviceToHost));
-	if (n64)
-		SCK(v, hipMemcpy(v->h_to64, v->L.to64, n64 * sizeof(int), hipMemcpyDeviceToHost));
-	for (int k = 0; k < 3; ++k)
-		v->steps[k] += v->h_ctr->steps[k];
-	for (unsigned i = 0; i < n64; ++i) {
-		int s = v->h_to64[i];
-		list_remove(v, s);
-		list_add(v, 1, s);
-	}
-	for (unsigned i = 0; i < nf; ++i) {
-		int s = v->h_fin[i];
-		list_remove(v, s);
-		HReq &r = v->reqs[v->slot_req[s]];
-		v->slot_req[s] = -1;
-		v->free_slots.push_back(s);
-		if (--r.remaining == 0) {
-			r.state = GRAYHIP_REQ_DONE;
-			v->free_tab.push_back(r.tab);
-			v->done.push_back(r.id);
-		}
-	}
-	return GRAYHIP_OK;
-}
-
-static int admit(grayhip_server *v)
-{
-	while (!v->pending.empty() && !v->free_tab.empty()) {
-		HReq &r = v->reqs[v->pending.front()];
-		size_t off;
-		if (!v->alloc.alloc(r.bytes, &off))
-			break; /* wait until finished requests release memory */
-		r.off = off;
-		r.tab = v->free_tab.back();
-		v->free_tab.pop_back();
-		double *base = (double *)(v->arena + off);
-		r.dev.out_data = base;
-		r.dev.out_info = base + r.n_rays * (size_t)r.nd;
-		r.dev.out_steps = (int64_t *)(r.dev.out_info + r.n_rays);
-		SCK(v, hipMemcpy(v->d_reqs + r.tab, &r.dev, sizeof(SrvRequest), hipMemcpyHostToDevice));
-		r.state = GRAYHIP_REQ_RUNNING;
-		v->filling.push_back(r.id);
-		v->pending.pop_front();
-	}
-	return GRAYHIP_OK;
-}
-
-/* Give free slots to rays of admitted requests.  The free slots are shared
- * evenly among the requests still waiting for slots (the oldest take the
- * remainder), so a small request that arrives behind a large one gets its
- * slots at the next epoch instead of after the large one's last ray. */
-static int refill(grayhip_server *v, int *n_assign)
-{
-	const size_t S = (size_t)v->S.S;
-	int n = 0;
-	int *as = v->h_assign, *ar = v->h_assign + S, *ay = v->h_assign + 2 * S;
-	while (!v->filling.empty() && !v->free_slots.empty()) {
-		const size_t n_req = v->filling.size();
-		const size_t share = v->free_slots.size() / n_req;
-		size_t extra = v->free_slots.size() % n_req;
-		for (size_t k = 0; k < n_req && !v->free_slots.empty(); ++k) {
-			HReq &r = v->reqs[v->filling[k]];
-			size_t give = share + (extra > 0 ? 1 : 0);
-			if (extra > 0)
-				--extra;
-			for (; give > 0 && r.next < r.n_rays && !v->free_slots.empty(); --give) {
-				int s = v->free_slots.back();
-				v->free_slots.pop_back();
-				as[n] = s;
-				ar[n] = r.tab;
-				ay[n] = (int)r.next++;
-				v->slot_req[s] = r.id;
-				list_add(v, r.kind == SRV_GRID ? 2 : 0, s);
-				++n;
-			}
-		}
-		/* drop requests whose rays all have slots; slots a full request
-		 * left unused go round again */
-		std::deque<int64_t> still;
-		for (int64_t id : v->filling)
-			if (v->reqs[id].next < v->reqs[id].n_rays)
-				still.push_back(id);
-		v->filling.swap(still);
-	}
-	*n_assign = n;
-	if (n) {
-		int *d = v->d_assign;
-		SCK(v, hipMemcpyAsync(d, as, n * sizeof(int), hipMemcpyHostToDevice, v->s[0]));
-		SCK(v, hipMemcpyAsync(d + S, ar, n * sizeof(int), hipMemcpyHostToDevice, v->s[0]));
-		SCK(v, hipMemcpyAsync(d + 2 * S, ay, n * sizeof(int), hipMemcpyHostToDevice, v->s[0]));
-	}
-	return GRAYHIP_OK;
+	return v->sched->submit(r, id, t_err, sizeof(t_err));
 }
 
 extern "C" int grayhip_server_step(grayhip_server *v, int *busy)
 {
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
-	SCK(v, hipSetDevice(v->cfg.device));
-	int n_assign = 0;
-	bool work;
-	{
-		std::lock_guard<std::mutex> lk(v->mu);
-		int rc = admit(v);
-		if (!rc)
-			rc = refill(v, &n_assign);
-		if (rc)
-			return rc;
-		work = !v->list[0].empty() || !v->list[1].empty() || !v->list[2].empty();
-		if (busy)
-			*busy = work || !v->pending.empty();
-	}
-	if (!work)
-		return GRAYHIP_OK;
-	const size_t S = (size_t)v->S.S;
-	SCK(v, hipMemsetAsync(v->L.ctr, 0, sizeof(SrvCounters), v->s[0]));
-	for (int k = 0; k < 3; ++k)
-		if (!v->list[k].empty())
-			SCK(v, hipMemcpyAsync(v->d_list[k], v->list[k].data(), v->list[k].size() * sizeof(int),
-			                      hipMemcpyHostToDevice, v->s[0]));
-	SCK(v, launch_srv_init(v->d_reqs, v->S, v->d_assign, v->d_assign + S, v->d_assign + 2 * S,
-	                       n_assign, v->s[0]));
-	SCK(v, hipEventRecord(v->ev_ready, v->s[0]));
-	SCK(v, hipStreamWaitEvent(v->s[1], v->ev_ready, 0));
-	SCK(v, hipStreamWaitEvent(v->s[2], v->ev_ready, 0));
-	const int K = v->cfg.epoch_steps;
-	SCK(v, launch_srv_epoch(0, v->d_reqs, v->S, v->d_list[0], (int)v->list[0].size(), K, v->L, v->s[0]));
-	SCK(v, launch_srv_epoch(1, v->d_reqs, v->S, v->d_list[1], (int)v->list[1].size(), K, v->L, v->s[1]));
-	SCK(v, launch_srv_epoch(2, v->d_reqs, v->S, v->d_list[2], (int)v->list[2].size(), K, v->L, v->s[2]));
-	for (int k = 0; k < 3; ++k)
-		SCK(v, hipStreamSynchronize(v->s[k]));
-	std::lock_guard<std::mutex> lk(v->mu);
-	++v->epochs;
-	return harvest(v);
+	return v->sched->step(busy, t_err, sizeof(t_err));
 }
 
 extern "C" int grayhip_server_next_done(grayhip_server *v, int64_t *id)
 {
 	if (!v || !id)
 		return vfail(v, GRAYHIP_EINVAL, "null argument");
-	std::lock_guard<std::mutex> lk(v->mu);
-	if (v->done.empty())
-		return 1;
-	*id = v->done.front();
-	v->done.pop_front();
-	return GRAYHIP_OK;
+	return v->sched->next_done(id);
 }
 
 extern "C" int grayhip_server_state(grayhip_server *v, int64_t id, int *state)
 {
 	if (!v || !state)
 		return vfail(v, GRAYHIP_EINVAL, "null argument");
-	std::lock_guard<std::mutex> lk(v->mu);
-	auto it = v->reqs.find(id);
-	if (it == v->reqs.end())
-		return vfail(v, GRAYHIP_EINVAL, "unknown request %lld", (long long)id);
-	*state = it->second.state;
+	int rc = v->sched->state(id, state);
+	if (rc)
+		return vfail(v, rc, "unknown request %lld", (long long)id);
 	return GRAYHIP_OK;
 }
 
@@ -497,35 +588,13 @@ extern "C" int grayhip_server_result(grayhip_server *v, int64_t id, double *data
 {
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
-	std::lock_guard<std::mutex> lk(v->mu);
-	auto it = v->reqs.find(id);
-	if (it == v->reqs.end())
-		return vfail(v, GRAYHIP_EINVAL, "unknown request %lld", (long long)id);
-	HReq &r = it->second;
-	if (r.state != GRAYHIP_REQ_DONE)
-		return vfail(v, GRAYHIP_ESTATE, "request %lld is not finished", (long long)id);
-	SCK(v, hipSetDevice(v->cfg.device));
-	if (data)
-		SCK(v, hipMemcpy(data, r.dev.out_data, r.n_rays * (size_t)r.nd * 8, hipMemcpyDeviceToHost));
-	if (info)
-		SCK(v, hipMemcpy(info, r.dev.out_info, r.n_rays * 8, hipMemcpyDeviceToHost));
-	if (steps)
-		SCK(v, hipMemcpy(steps, r.dev.out_steps, r.n_rays * 8, hipMemcpyDeviceToHost));
-	v->alloc.release(r.off, r.bytes);
-	v->done.erase(std::remove(v->done.begin(), v->done.end(), id), v->done.end());
-	v->reqs.erase(it);
-	return GRAYHIP_OK;
+	return v->sched->result(id, data, info, steps, t_err, sizeof(t_err));
 }
 
 extern "C" int grayhip_server_stats(grayhip_server *v, int64_t steps[3], int64_t *epochs)
 {
 	if (!v)
 		return vfail(nullptr, GRAYHIP_EINVAL, "null server");
-	std::lock_guard<std::mutex> lk(v->mu);
-	if (steps)
-		for (int k = 0; k < 3; ++k)
-			steps[k] = (int64_t)v->steps[k];
-	if (epochs)
-		*epochs = v->epochs;
+	v->sched->stats(steps, epochs);
 	return GRAYHIP_OK;
 }

@@ -1,23 +1,28 @@
 /*
- * kernels_serve.hip -- epoch kernels of the serving pool.
+ * kernels_serve.hip -- the serving pool's epoch kernels (serve.h).
  *
- * Every kernel takes a compacted list of slots, one lane per slot, and
- * advances each slot's ray by at most K RK4 steps (or to its end), so an
- * epoch is bounded and new requests get slots at the next epoch.  The FP32
- * state crosses epochs as separate hi and lo floats, which is exact: a ray
- * computes the same numbers whatever epochs, slots or neighbours it gets.
+ *   k_srv_kerr   analytic Kerr, FP32 form: a persistent ray pool whose
+ *                blocks are shared out among the Kerr classes; a wave
+ *                refills its idle lanes from its class's queue with one
+ *                ballot and one atomicAdd (resumed rays first, then fresh
+ *                rays, whose camera initial conditions it computes in FP64);
+ *                collapsed-step and non-finite rays go to the FP64 list
+ *   k_srv_fb64   FP64 re-integration of that list (the step of step_fb.h,
+ *                as the render's re-integration); survivors are appended to
+ *                the next epoch's list, beside the Kerr kernel's hand-overs
+ *   k_srv_grid   gridded spacetimes, FP32 form, one 8x8-ray tile per wave
+ *                with the wave's LDS brick (grid_brick32.h), transfer
+ *                channels of the reference flow on the scaled state
  *
- *   k_srv_init    camera initial conditions in FP64 for newly assigned slots
- *   k_srv_kerr32  analytic Kerr, FP32 form (per-lane spin, channels, camera)
- *   k_srv_kerr64  analytic Kerr, FP64 re-integration of collapsed-step rays
- *                 (the step of step_fb.h, as the render's re-integration)
- *   k_srv_grid32  gridded spacetimes, FP32 form
- *
- * Built with multiply-add contraction (these are FP32-form kernels).
+ * Every wave runs at most K iterations, then saves what it still holds to
+ * the resume lists, so an epoch is bounded and new requests enter at the
+ * next one.  Saved states are the FP32 (hi, lo) pairs and integers as they
+ * were: a ray computes the same numbers whatever epochs, lanes or
+ * neighbours it gets.  Built with multiply-add contraction (FP32-form code).
  */
 #include <climits>
 
-#include "grid_f32.h"
+#include "grid_brick32.h"
 #include "grid_f64.h"
 #include "launch_serve.h"
 #include "step_f32.h"
@@ -26,232 +31,256 @@
 
 namespace gray {
 
-__device__ __forceinline__ LaneKerr lane_params(const RayParams &p)
-{
-	LaneKerr L;
-	L.a = (float)p.a;
-	L.r_h = (float)p.r_h;
-	L.eps_stop = (float)p.eps_stop;
-	L.r_esc = (float)p.r_esc;
-	L.r_isco = (float)p.r_isco;
-	L.escape = p.escape;
-	L.fluid = p.fluid;
-	L.n_freq = p.n_freq;
-#pragma unroll
-	for (int i = 0; i < SRV_NF; ++i)
-		L.nu[i] = (float)p.nu[i < p.n_freq ? i : 0];
-	return L;
-}
+/* What the FP32 Kerr step reads (getdt, rt_rhs_f32): spin-derived values
+ * per lane, the class's escape rule and fluid per block. */
+struct LaneKerr {
+	float a, r_h, eps_stop, r_isco, r_esc;
+	int escape, fluid;
+};
 
-/* Write a finished ray into its request's output: geodesic state, the
- * request's channels (I then tau), g(u,u) and the step count. */
-__device__ __forceinline__ void write_out(const SrvRequest &R, int ray, const double *s, int nf_slot,
-                                          double info, int64_t steps)
+/* A finished ray into its request's output: geodesic state, the request's
+ * channels (I, then tau: x holds I at 8 and tau at 8 + nf_slot), g(u,u), the
+ * step count; the request is reported when its last ray is in. */
+__device__ __forceinline__ void finish_ray(const SrvRequest &R, int tab, int ray, const double *x,
+                                           int nf_slot, double info, int64_t steps,
+                                           unsigned *remaining, SrvCtr *ctr, int *done)
 {
-	double *o = R.out_data + (size_t)ray * R.nd;
+	double *o = R.out_data + (size_t)ray * (size_t)R.nd;
 #pragma unroll
 	for (int i = 0; i < 8; ++i)
-		o[i] = s[i];
+		o[i] = x[i];
 	const int nf = R.p.n_freq;
 	for (int i = 0; i < nf; ++i) {
-		o[8 + i] = s[8 + i];
-		o[8 + nf + i] = s[8 + nf_slot + i];
+		o[8 + i] = x[8 + i];
+		o[8 + nf + i] = x[8 + nf_slot + i];
 	}
 	R.out_info[ray] = info;
 	R.out_steps[ray] = steps;
+	if (atomicSub(&remaining[tab], 1u) == 1u)
+		done[atomicAdd(&ctr->n_done, 1u)] = tab;
 }
 
-__device__ __forceinline__ void push(int *list, unsigned *count, int v)
+/* Queue item f of the fresh part: segment j with start[j] <= f <
+ * start[j + 1] in [s0, s1) (binary search), item first[j] + f - start[j]. */
+__device__ __forceinline__ void seg_item(const SrvSegs &S, int s0, int s1, unsigned f, int &tab,
+                                         int &first)
 {
-	unsigned k = atomicAdd(count, 1u);
-	list[k] = v;
+	int lo = s0, hi = s1 - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (S.start[mid] <= f)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	tab = S.tab[lo];
+	first = S.first[lo] + (int)(f - S.start[lo]);
 }
 
-__global__ __launch_bounds__(256) void k_srv_init(const SrvRequest *__restrict__ reqs, SrvSlots S,
-                                                  const int *__restrict__ a_slot,
-                                                  const int *__restrict__ a_req,
-                                                  const int *__restrict__ a_ray, int n)
-{
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	const int s = a_slot[i], r = a_req[i], g = a_ray[i];
-	const SrvRequest &R = reqs[r];
-	double st[SRV_NC];
-#pragma unroll
-	for (int c = 0; c < SRV_NC; ++c)
-		st[c] = 0.0;
-	if (R.kind == SRV_KERR) {
-		icond_ray<SRV_NF, RayParams>(R.p, (size_t)g, st);
-	} else {
-		/* camera + null normalisation with the interpolated metric */
-		const RayParams &p = R.p;
-		size_t gi = (size_t)g % (size_t)p.w_rays, gj = (size_t)g / (size_t)p.w_rays;
-		double alpha = (((double)gi + 0.5) / (double)p.w_rays - 0.5) * p.w_img;
-		double beta  = (((double)gj + 0.5) / (double)p.h_rays - 0.5) * p.h_img;
-		double R0 = p.r_obs * p.si - beta * p.ci;
-		st[0] = 0.0;
-		st[3] = p.r_obs * p.ci + beta * p.si;
-		st[2] = R0 * p.sj + alpha * p.cj;
-		st[1] = R0 * p.cj - alpha * p.sj;
-		st[4] = 1.0;
-		st[5] = p.si * p.cj;
-		st[6] = p.si * p.sj;
-		st[7] = p.ci;
-		Stencil<double> sc;
-		make_stencil<double, float>(R.gv, st[0], st[1], st[2], st[3], sc);
-		double m[10];
-		sample_f64<10, 10, float>(R.gv.met, 0, sc, m);
-		double A = m[0];
-		double B = ((m[1] * st[5] + m[2] * st[6]) + m[3] * st[7]) * 2.0;
-		double C = ((((m[4] * st[5] + m[5] * st[6]) + m[6] * st[7]) * st[5]) +
-		            (((m[5] * st[5] + m[7] * st[6]) + m[8] * st[7]) * st[6])) +
-		            (((m[6] * st[5] + m[8] * st[6]) + m[9] * st[7]) * st[7]);
-		double k = (-(B + sqrt(B * B - (4.0 * A) * C))) / (2.0 * A);
-		st[5] /= k;
-		st[6] /= k;
-		st[7] /= k;
-	}
-#pragma unroll
-	for (int c = 0; c < SRV_NC; ++c) {
-		float h = (float)st[c];
-		S.st[(size_t)c * S.S + s] = h;
-		S.st[(size_t)(SRV_NC + c) * S.S + s] = (float)(st[c] - (double)h);
-	}
-	S.req[s] = r;
-	S.ray[s] = g;
-	S.d[s] = 0;
-	S.h[s] = 0;
-	S.steps[s] = 0;
-}
+/* ---------------- analytic Kerr, FP32 form ---------------- */
 
-__global__ __launch_bounds__(256) void k_srv_kerr32(const SrvRequest *__restrict__ reqs, SrvSlots S,
-                                                    const int *__restrict__ list, int n, int K,
-                                                    SrvLists out)
+template <int NF>
+__device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A, const SrvClass &C, int c)
 {
-	int idx = blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= n)
-		return;
-	const int s = list[idx];
-	const SrvRequest &R = reqs[S.req[s]];
-	const LaneKerr P = lane_params(R.p);
-	const RtConst<SRV_NF> rc = make_rtconst<SRV_NF>(P);
-	const StepRule sr = R.sr;
-	/* the slot holds the FP32 form's state (geodesic, intensities, the
-	 * shared depth integral T in channel 0's depth place; step_f32.h) */
-	constexpr int NI = nd32<SRV_NF>();
-	float hi[NI], lo[NI];
-#pragma unroll
-	for (int c = 0; c < NI; ++c) {
-		hi[c] = S.st[(size_t)c * S.S + s];
-		lo[c] = S.st[(size_t)(SRV_NC + c) * S.S + s];
-	}
-	int d = S.d[s], h = S.h[s];
-	int64_t st = S.steps[s];
-	const int64_t st0 = st;
+	constexpr int ND = nd32<NF>();
+	const unsigned lane = __lane_id();
+	const RtConst<NF> rc = make_rtconst<NF>(C);
+	const StepRule sr = C.sr;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
 	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
-	int state = 0; /* 0 running, 1 finished, 2 to FP64, 3 restart in FP64 */
-	for (int k = 0; k < K; ++k) {
-		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
-		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
-		float gdt = getdt(P, q, u, dts);
-		double xd = (double)gdt * n_over_dt;
-		int dh = (xd == xd) ? (int)xd : 0;
-		if (dh == 0) {
-			state = 1;
-			break;
-		}
-		if (dh > sr.n - h)
-			dh = sr.n - h;
-		double hd = (double)dh * dt_over_n;
-		if (fabs(hd) < 1e-4) {
-			state = 2;
-			break;
-		}
-		rk4_f32c<SRV_NF>(P, rc, hi, lo, (float)hd);
-		++st;
-		float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
-		if (!__builtin_isfinite(chk)) {
-			state = 3;
-			break;
-		}
-		h += dh;
-		if (h >= sr.n) {
-			h = 0;
-			if (++d >= sr.n_dump) {
-				state = 1;
-				break;
+	const unsigned items = C.items;
+	const int n_resume = C.n_resume;
+	LaneKerr P;
+	P.r_esc = (float)C.r_esc;
+	P.escape = C.escape;
+	P.fluid = C.fluid;
+	P.a = P.r_h = P.eps_stop = P.r_isco = 0.f;
+	float hi[ND], lo[ND];
+	int tab = -1, ray = 0, d = 0, h = 0;
+	int64_t st = 0, st0 = 0;
+	bool exhausted = false;
+	for (int it = 0;; ++it) {
+		const bool stop = it >= A.K;
+		const bool need = tab < 0 && !exhausted && !stop;
+		const unsigned long long m = __ballot(need);
+		if (m) {
+			const int leader = __ffsll((unsigned long long)m) - 1;
+			unsigned base = 0;
+			if ((int)lane == leader)
+				base = atomicAdd(&A.ctr->claimed[c], (unsigned)__popcll(m));
+			base = __shfl(base, leader);
+			if (need) {
+				const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+				if (idx >= items) {
+					exhausted = true;
+				} else if ((int)idx < n_resume) {
+					const int e = C.res_off + (int)idx;
+					const int4 mt = A.in.meta[e];
+					tab = mt.x;
+					ray = mt.y;
+					d = mt.z;
+					h = mt.w;
+					st = A.in.steps[e];
+#pragma unroll
+					for (int i = 0; i < ND; ++i) {
+						hi[i] = A.in.st[(size_t)i * A.in.cap + e];
+						lo[i] = A.in.st[(size_t)(SRV_NK + i) * A.in.cap + e];
+					}
+				} else {
+					seg_item(A.seg, C.seg0, C.seg1, idx - (unsigned)n_resume, tab, ray);
+					double x[8 + 2 * NF];
+					icond_ray<NF, RayParams>(A.reqs[tab].p, (size_t)ray, x);
+					split_state<NF, SrvClass>(C, x, hi, lo);
+					d = 0;
+					h = 0;
+					st = 0;
+				}
+				if (tab >= 0) {
+					const RayParams &p = A.reqs[tab].p;
+					P.a = (float)p.a;
+					P.r_h = (float)p.r_h;
+					P.eps_stop = (float)p.eps_stop;
+					P.r_isco = (float)p.r_isco;
+					st0 = st;
+				}
 			}
 		}
-	}
-	atomicAdd(&out.ctr->steps[0], (unsigned long long)(st - st0));
-	if (state == 0) {
+		if (tab < 0)
+			break;
+		if (stop) {
+			/* iterations used up: the ray goes on next epoch */
+			const int e = C.block0 * SRV_BLOCK + (int)atomicAdd(&A.ctr->saved[c], 1u);
+			A.out.meta[e] = make_int4(tab, ray, d, h);
+			A.out.steps[e] = st;
 #pragma unroll
-		for (int c = 0; c < NI; ++c) {
-			S.st[(size_t)c * S.S + s] = hi[c];
-			S.st[(size_t)(SRV_NC + c) * S.S + s] = lo[c];
+			for (int i = 0; i < ND; ++i) {
+				A.out.st[(size_t)i * A.out.cap + e] = hi[i];
+				A.out.st[(size_t)(SRV_NK + i) * A.out.cap + e] = lo[i];
+			}
+			atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
+			break;
 		}
-		S.d[s] = d;
-		S.h[s] = h;
-		S.steps[s] = st;
-		return;
-	}
-	double x[SRV_NC];
-	join_state<SRV_NF>(R.p, hi, lo, x);
-	if (state == 1) {
-		V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
-		write_out(R, S.ray[s], x, SRV_NF, getuu(R.p.a, qd, ud), st);
-		push(out.fin, &out.ctr->n_fin, s);
-		return;
-	}
+		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
+		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
+		const float gdt = getdt(P, q, u, dts);
+		const double xd = (double)gdt * n_over_dt;
+		int dh = (xd == xd) ? (int)xd : 0;
+		int state = 0;   /* 1 finished, 2 to FP64 here, 3 FP64 from the camera */
+		if (dh == 0) {
+			state = 1;
+		} else {
+			if (dh > sr.n - h)
+				dh = sr.n - h;
+			const double hd = (double)dh * dt_over_n;
+			if (fabs(hd) < 1e-4) {
+				state = 2;
+			} else {
+				rk4_f32c<NF>(P, rc, hi, lo, (float)hd);
+				++st;
+				const float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
+				if (!__builtin_isfinite(chk)) {
+					state = 3;
+				} else {
+					h += dh;
+					if (h >= sr.n) {
+						h = 0;
+						if (++d >= sr.n_dump)
+							state = 1;
+					}
+				}
+			}
+		}
+		if (state == 0)
+			continue;
+		atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
+		const SrvRequest &R = A.reqs[tab];
+		if (state == 1) {
+			double x[8 + 2 * NF];
+			join_state<NF, SrvClass>(C, hi, lo, x);
+			V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
+			finish_ray(R, tab, ray, x, NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr, A.done);
+		} else {
+			/* the FP64 list's layout: geodesic, SRV_NF intensities (unused
+			 * ones 0), the shared depth T itself (converted to the channels'
+			 * depths only when the ray ends, so that epochs in FP64 round
+			 * nothing) */
+			double x[SRV_NC];
 #pragma unroll
-	for (int c = 0; c < SRV_NC; ++c)
-		S.st64[(size_t)c * S.S + s] = x[c];
-	S.d[s] = d;
-	S.h[s] = state == 3 ? -1 : h;
-	S.steps[s] = st;
-	push(out.to64, &out.ctr->n_to64, s);
+			for (int i = 0; i < 8 + NF; ++i)
+				x[i] = (double)hi[i] + (double)lo[i];
+#pragma unroll
+			for (int i = 8 + NF; i < SRV_NC; ++i)
+				x[i] = 0.0;
+			x[8 + SRV_NF] = NF > 0 ? (double)hi[ND - 1] + (double)lo[ND - 1] : 0.0;
+			/* the host plans the list's room; a count past it is reported
+			 * as an error and nothing is written out of bounds */
+			const unsigned e = atomicAdd(&A.ctr->fb_out, 1u);
+			if (e < (unsigned)A.fb.cap) {
+				A.fb.meta[e] = make_int4(tab, ray, d, state == 3 ? -1 : h);
+				A.fb.steps[e] = st;
+#pragma unroll
+				for (int i = 0; i < SRV_NC; ++i)
+					A.fb.x[(size_t)i * A.fb.cap + e] = x[i];
+			}
+		}
+		tab = -1;
+	}
 }
 
-__global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict__ reqs, SrvSlots S,
-                                                    const int *__restrict__ list, int n, int K,
-                                                    SrvLists out)
+__global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A)
 {
-	int idx = blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= n)
+	const int b = (int)blockIdx.x;
+	for (int c = 0; c < A.n_cls; ++c) {
+		const SrvClass &C = A.cls[c];
+		if (b < C.block0 || b >= C.block1)
+			continue;
+		switch (C.n_freq) {
+		case 0: kerr_lanes<0>(A, C, c); break;
+		case 1: kerr_lanes<1>(A, C, c); break;
+		case 2: kerr_lanes<2>(A, C, c); break;
+		case 3: kerr_lanes<3>(A, C, c); break;
+		default: kerr_lanes<4>(A, C, c); break;
+		}
 		return;
-	const int s = list[idx];
-	const SrvRequest &R = reqs[S.req[s]];
+	}
+}
+
+/* ---------------- FP64 re-integration ---------------- */
+
+__global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A)
+{
+	const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e >= A.n_in)
+		return;
+	const int4 mt = A.in.meta[e];
+	const int tab = mt.x, ray = mt.y;
+	int d = mt.z, h = mt.w;
+	const SrvRequest &R = A.reqs[tab];
 	const StepRule sr = R.sr;
 	const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
 	double x[SRV_NC];
-	int d = S.d[s], h = S.h[s];
-	int64_t st = S.steps[s];
+	int64_t st = A.in.steps[e];
 	if (h < 0) {
 		/* non-finite FP32 state: the whole ray again in FP64 */
-		icond_ray<SRV_NF, RayParams>(R.p, (size_t)S.ray[s], x);
+		icond_ray<SRV_NF, RayParams>(R.p, (size_t)ray, x);
 		d = 0;
 		h = 0;
 		st = 0;
 	} else {
 #pragma unroll
 		for (int c = 0; c < SRV_NC; ++c)
-			x[c] = S.st64[(size_t)c * S.S + s];
+			x[c] = A.in.x[(size_t)c * A.in.cap + e];
 	}
-	/* x holds the stored layout (tau_i in place); the step integrates the
-	 * channels' shared depth T = tau_0 / d_0 in tau_0's place */
-	x[8 + SRV_NF] = x[8 + SRV_NF] / rc.d[0];
+	/* x[8 + SRV_NF] holds the channels' shared depth T (0 from the camera) */
 	const int64_t st0 = st;
 	const double n_over_dt = (double)sr.n / sr.dt;
 	bool fin = false;
-	int dh;
-	for (int k = 0; k < K; ++k) {
+	for (int k = 0; k < A.K; ++k) {
 		V4<double> q = {x[0], x[1], x[2], x[3]};
 		V4<double> u = {x[4], x[5], x[6], x[7]};
-		double xd = getdt(R.p, q, u, sr.dts) * n_over_dt;
-		dh = (xd == xd) ? (int)xd : 0;
+		const double xd = getdt(R.p, q, u, sr.dts) * n_over_dt;
+		int dh = (xd == xd) ? (int)xd : 0;
 		if (!dh) {
 			fin = true;
 			break;
@@ -269,122 +298,282 @@ __global__ __launch_bounds__(256) void k_srv_kerr64(const SrvRequest *__restrict
 			}
 		}
 	}
-	atomicAdd(&out.ctr->steps[1], (unsigned long long)(st - st0));
-	{
+	atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
+	if (fin) {
 		const double T = x[8 + SRV_NF];
 #pragma unroll
 		for (int i = 0; i < SRV_NF; ++i)
 			x[8 + SRV_NF + i] = rc.d[i] * T;
-	}
-	if (fin) {
 		V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
-		write_out(R, S.ray[s], x, SRV_NF, getuu(R.p.a, qd, ud), st);
-		push(out.fin, &out.ctr->n_fin, s);
+		finish_ray(R, tab, ray, x, SRV_NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr, A.done);
 		return;
 	}
+	const unsigned o = atomicAdd(&A.ctr->fb_out, 1u);
+	if (o >= (unsigned)A.out.cap)
+		return;   /* reported by the host as an overflow */
+	A.out.meta[o] = make_int4(tab, ray, d, h);
+	A.out.steps[o] = st;
 #pragma unroll
 	for (int c = 0; c < SRV_NC; ++c)
-		S.st64[(size_t)c * S.S + s] = x[c];
-	S.d[s] = d;
-	S.h[s] = h;
-	S.steps[s] = st;
+		A.out.x[(size_t)c * A.out.cap + o] = x[c];
 }
 
-__global__ __launch_bounds__(256) void k_srv_grid32(const SrvRequest *__restrict__ reqs, SrvSlots S,
-                                                    const int *__restrict__ list, int n, int K,
-                                                    SrvLists out)
+/* ---------------- gridded spacetimes, FP32 form ---------------- */
+
+/* Camera and null normalisation with the interpolated metric in FP64 (the
+ * render's k_grid_icond). */
+__device__ __forceinline__ void grid_icond_ray(const SrvRequest &R, int g, double *s)
 {
-	int idx = blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= n)
-		return;
-	const int s = list[idx];
-	const SrvRequest &R = reqs[S.req[s]];
+	const RayParams &p = R.p;
+	const int gi = g % p.w_rays, gj = g / p.w_rays;
+	const double alpha = (((double)gi + 0.5) / (double)p.w_rays - 0.5) * p.w_img;
+	const double beta = (((double)gj + 0.5) / (double)p.h_rays - 0.5) * p.h_img;
+	const double R0 = p.r_obs * p.si - beta * p.ci;
+	s[0] = 0.0;
+	s[3] = p.r_obs * p.ci + beta * p.si;
+	s[2] = R0 * p.sj + alpha * p.cj;
+	s[1] = R0 * p.cj - alpha * p.sj;
+	s[4] = 1.0;
+	s[5] = p.si * p.cj;
+	s[6] = p.si * p.sj;
+	s[7] = p.ci;
+	if (!R.gv.raw_icond) {
+		Stencil<double> sc;
+		make_stencil<double, float>(R.gv, s[0], s[1], s[2], s[3], sc);
+		double m[10];
+		sample_f64<10, 10, float>(R.gv.met, 0, sc, m);
+		const double A = m[0];
+		const double B = ((m[1] * s[5] + m[2] * s[6]) + m[3] * s[7]) * 2.0;
+		const double Cc = ((((m[4] * s[5] + m[5] * s[6]) + m[6] * s[7]) * s[5]) +
+		                   (((m[5] * s[5] + m[7] * s[6]) + m[8] * s[7]) * s[6])) +
+		                  (((m[6] * s[5] + m[8] * s[6]) + m[9] * s[7]) * s[7]);
+		const double k = (-(B + sqrt(B * B - (4.0 * A) * Cc))) / (2.0 * A);
+		s[5] = s[5] / k;
+		s[6] = s[6] / k;
+		s[7] = s[7] / k;
+	}
+}
+
+/* One tile of a gridded request from its current state until its rays are
+ * done or the wave's iterations run out; then saves the live lanes.
+ * Returns the iterations used. */
+template <int NF>
+__device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile, bool resumed,
+                                         int rec, float4 *brick, BrickHdr *hdr, int it)
+{
+	constexpr int ND = 8 + 2 * NF;
+	const unsigned lane = __lane_id();
+	const SrvRequest &R = A.reqs[tab];
 	const GridView<float> &G = R.gv;
 	const StepRule sr = R.sr;
-	float hi[8], lo[8];
+	const RtThermalF rt = rt_thermal_f32(R.rt);
+	const double scale = R.rt.scale;
+	float hi[ND > 0 ? ND : 1], lo[ND > 0 ? ND : 1];
+	int ray = -1, d = 0, h = 0;
+	int64_t st = 0;
+	if (resumed) {
+		const int e = rec * 64 + (int)lane;
+		const int4 mt = A.in.meta[e];
+		ray = mt.x;
+		d = mt.y;
+		h = mt.z;
+		st = A.in.steps[e];
+		if (ray >= 0) {
 #pragma unroll
-	for (int c = 0; c < 8; ++c) {
-		hi[c] = S.st[(size_t)c * S.S + s];
-		lo[c] = S.st[(size_t)(SRV_NC + c) * S.S + s];
-	}
-	int d = S.d[s], h = S.h[s];
-	int64_t st = S.steps[s];
-	const int64_t st0 = st;
-	const float dts = (float)sr.dts;
-	const double dt_over_n = sr.dt / (double)sr.n;
-	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
-	bool fin = false;
-	for (int k = 0; k < K; ++k) {
-		float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], hi[5], hi[6], hi[7], dts);
-		double xd = (double)gdt * n_over_dt;
-		int dh = (xd == xd) ? (int)xd : 0;
-		if (!dh) {
-			fin = true;
-			break;
+			for (int i = 0; i < ND; ++i) {
+				hi[i] = A.in.st[(size_t)i * (64 * (size_t)A.in.cap) + e];
+				lo[i] = A.in.st[(size_t)(SRV_NG + i) * (64 * (size_t)A.in.cap) + e];
+			}
 		}
-		if (dh > sr.n - h)
-			dh = sr.n - h;
-		grid_rk4_f32c<0, float>(G, RtThermalF{}, hi, lo, (float)((double)dh * dt_over_n));
-		++st;
-		h += dh;
-		if (h >= sr.n) {
-			h = 0;
-			if (++d >= sr.n_dump) {
-				fin = true;
-				break;
+	} else {
+		const int tx = tile % R.tw, ty = tile / R.tw;
+		const int x = tx * 8 + (int)(lane & 7u), y = ty * 8 + (int)(lane >> 3);
+		if (x < R.p.w_rays && y < R.p.h_rays) {
+			ray = y * R.p.w_rays + x;
+			double s[8];
+			grid_icond_ray(R, ray, s);
+#pragma unroll
+			for (int i = 0; i < ND; ++i) {
+				const double v = i < 8 ? s[i] : 0.0;
+				hi[i] = (float)v;
+				lo[i] = (float)(v - (double)hi[i]);
 			}
 		}
 	}
-	atomicAdd(&out.ctr->steps[2], (unsigned long long)(st - st0));
-	if (fin) {
-		double x[SRV_NC];
-#pragma unroll
-		for (int c = 0; c < 8; ++c)
-			x[c] = (double)hi[c] + (double)lo[c];
-#pragma unroll
-		for (int c = 8; c < SRV_NC; ++c)
-			x[c] = 0.0;
-		write_out(R, S.ray[s], x, SRV_NF, grid_getuu_f64<float>(G, x), st);
-		push(out.fin, &out.ctr->n_fin, s);
-		return;
+	/* a fresh tile starts with an empty brick (its first evaluation loads);
+	 * a resumed one with the brick it had when it was saved */
+	const bool use_lds = G.n[0] >= GB && G.n[1] >= GB && G.n[2] >= GB;
+	int4 bo = make_int4(-1000000, -1000000, -1000000, 0);
+	if (resumed)
+		bo = A.in.brick[rec];
+	if (lane == 0) {
+		hdr->o[0] = bo.x;
+		hdr->o[1] = bo.y;
+		hdr->o[2] = bo.z;
+		hdr->cool = bo.w;
 	}
-#pragma unroll
-	for (int c = 0; c < 8; ++c) {
-		S.st[(size_t)c * S.S + s] = hi[c];
-		S.st[(size_t)(SRV_NC + c) * S.S + s] = lo[c];
+	if (use_lds && bo.x >= 0) {
+		const uint32_t nx = (uint32_t)G.n[0], nxy = nx * (uint32_t)G.n[1];
+		for (int q = (int)lane; q < GP_PTS; q += 64) {
+			const uint32_t pt = (uint32_t)(bo.z + q / (GB * GB)) * nxy +
+			                    (uint32_t)(bo.y + (q / GB) % GB) * nx + (uint32_t)(bo.x + q % GB);
+			brick_store<float>(brick, q, G.gam[0] + (size_t)pt * 40);
+		}
 	}
-	S.d[s] = d;
-	S.h[s] = h;
-	S.steps[s] = st;
+	wave_sync();
+	const float dts = (float)sr.dts;
+	const double dt_over_n = sr.dt / (double)sr.n;
+	const double n_over_dt = (double)sr.n / sr.dt;
+	const int64_t st0 = st;
+	bool live = ray >= 0;
+	for (; it < A.K; ++it) {
+		if (!__ballot(live))
+			break;
+		if (live) {
+			const float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], hi[5], hi[6],
+			                                           hi[7], dts);
+			const double xd = (double)gdt * n_over_dt;
+			int dh = (xd == xd) ? (int)xd : 0;
+			bool fin = !dh;
+			if (!fin) {
+				if (dh > sr.n - h)
+					dh = sr.n - h;
+				grid_rk4_brick<NF, float>(G, rt, brick, hdr, use_lds, hi, lo,
+				                          (float)((double)dh * dt_over_n));
+				++st;
+				h += dh;
+				if (h >= sr.n) {
+					h = 0;
+					fin = ++d >= sr.n_dump;
+				}
+			}
+			if (fin) {
+				double x[ND];
+#pragma unroll
+				for (int i = 0; i < ND; ++i) {
+					double v = (double)hi[i] + (double)lo[i];
+					x[i] = i >= 8 ? v * scale : v;
+				}
+				finish_ray(R, tab, ray, x, NF, grid_getuu_f64<float>(G, x), st, A.remaining,
+				           A.ctr, A.done);
+				live = false;
+				ray = -1;
+			}
+		}
+	}
+	atomicAdd(&A.ctr->steps[2], (unsigned long long)(st - st0));
+	wave_sync();   /* the brick header as the last evaluation left it */
+	if (__ballot(live)) {
+		/* iterations used up: the tile goes on next epoch */
+		unsigned r = 0;
+		if (lane == 0)
+			r = atomicAdd(&A.ctr->g_saved, 1u);
+		r = __shfl(r, 0);
+		const int e = (int)r * 64 + (int)lane;
+		if (lane == 0) {
+			A.out.hdr[r] = make_int2(tab, tile);
+			A.out.brick[r] = make_int4(hdr->o[0], hdr->o[1], hdr->o[2], hdr->cool);
+		}
+		A.out.meta[e] = make_int4(live ? ray : -1, d, h, 0);
+		A.out.steps[e] = st;
+		if (live) {
+#pragma unroll
+			for (int i = 0; i < ND; ++i) {
+				A.out.st[(size_t)i * (64 * (size_t)A.out.cap) + e] = hi[i];
+				A.out.st[(size_t)(SRV_NG + i) * (64 * (size_t)A.out.cap) + e] = lo[i];
+			}
+		}
+	}
+	return it;
 }
+
+__global__ __launch_bounds__(64 * SRV_GWAVES) void k_srv_grid(SrvGridArgs A)
+{
+	__shared__ float4 s_brick[SRV_GWAVES][GP_PTS * 10];
+	__shared__ BrickHdr s_hdr[SRV_GWAVES];
+	const unsigned lane = __lane_id();
+	const int wv = (int)(threadIdx.x >> 6);
+	float4 *brick = s_brick[wv];
+	BrickHdr *hdr = &s_hdr[wv];
+	int it = 0;
+	while (it < A.K) {
+		unsigned item = 0;
+		if (lane == 0)
+			item = atomicAdd(&A.ctr->g_claimed, 1u);
+		item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+		if (item >= A.items)
+			break;
+		int tab, tile, rec = -1;
+		const bool resumed = (int)item < A.n_resume;
+		if (resumed) {
+			rec = (int)item;
+			const int2 hd = A.in.hdr[rec];
+			tab = hd.x;
+			tile = hd.y;
+		} else {
+			seg_item(A.seg, A.seg0, A.seg1, item - (unsigned)A.n_resume, tab, tile);
+		}
+		tab = __builtin_amdgcn_readfirstlane(tab);
+		tile = __builtin_amdgcn_readfirstlane(tile);
+		switch (A.reqs[tab].p.n_freq) {
+		case 0: it = grid_tile<0>(A, tab, tile, resumed, rec, brick, hdr, it); break;
+		case 1: it = grid_tile<1>(A, tab, tile, resumed, rec, brick, hdr, it); break;
+		case 2: it = grid_tile<2>(A, tab, tile, resumed, rec, brick, hdr, it); break;
+		case 3: it = grid_tile<3>(A, tab, tile, resumed, rec, brick, hdr, it); break;
+		default: it = grid_tile<4>(A, tab, tile, resumed, rec, brick, hdr, it); break;
+		}
+	}
+}
+
+__global__ void k_srv_clear(SrvCtr *ctr)
+{
+	if (threadIdx.x == 0)
+		*ctr = SrvCtr{};
+}
+
+/* ---------------- launchers ---------------- */
 
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t launch_srv_init(const SrvRequest *reqs, const SrvSlots &S, const int *a_slot,
-                           const int *a_req, const int *a_ray, int n, hipStream_t st)
+hipError_t launch_srv_clear(SrvCtr *ctr, hipStream_t st)
 {
-	if (n <= 0)
-		return hipSuccess;
-	hipLaunchKernelGGL(k_srv_init, dim3(blocks_for(n, 256)), dim3(256), 0, st, reqs, S, a_slot,
-	                   a_req, a_ray, n);
+	hipLaunchKernelGGL(k_srv_clear, dim3(1), dim3(64), 0, st, ctr);
 	return hipGetLastError();
 }
 
-hipError_t launch_srv_epoch(int which, const SrvRequest *reqs, const SrvSlots &S, const int *list,
-                            int n, int K, const SrvLists &out, hipStream_t st)
+hipError_t launch_srv_kerr(const SrvKerrArgs &A, int n_blocks, hipStream_t st)
 {
-	if (n <= 0)
+	if (n_blocks <= 0 || A.n_cls <= 0)
 		return hipSuccess;
-	if (which == 0)
-		hipLaunchKernelGGL(k_srv_kerr32, dim3(blocks_for(n, 256)), dim3(256), 0, st, reqs, S, list,
-		                   n, K, out);
-	else if (which == 1)
-		hipLaunchKernelGGL(k_srv_kerr64, dim3(blocks_for(n, 256)), dim3(256), 0, st, reqs, S, list,
-		                   n, K, out);
-	else
-		hipLaunchKernelGGL(k_srv_grid32, dim3(blocks_for(n, 256)), dim3(256), 0, st, reqs, S, list,
-		                   n, K, out);
+	hipLaunchKernelGGL(k_srv_kerr, dim3(n_blocks), dim3(SRV_BLOCK), 0, st, A);
 	return hipGetLastError();
+}
+
+hipError_t launch_srv_fb64(const SrvFbArgs &A, hipStream_t st)
+{
+	if (A.n_in <= 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(k_srv_fb64, dim3(blocks_for((size_t)A.n_in, SRV_BLOCK)), dim3(SRV_BLOCK),
+	                   0, st, A);
+	return hipGetLastError();
+}
+
+hipError_t launch_srv_grid(const SrvGridArgs &A, int n_blocks, hipStream_t st)
+{
+	if (n_blocks <= 0 || A.items == 0)
+		return hipSuccess;
+	hipLaunchKernelGGL(k_srv_grid, dim3(n_blocks), dim3(64 * SRV_GWAVES), 0, st, A);
+	return hipGetLastError();
+}
+
+hipError_t srv_kerr_blocks_per_cu(int *blocks)
+{
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_srv_kerr, SRV_BLOCK, 0);
+}
+
+hipError_t srv_grid_blocks_per_cu(int *blocks)
+{
+	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_srv_grid, 64 * SRV_GWAVES, 0);
 }
 
 } // namespace gray

@@ -1,5 +1,6 @@
 /*
- * launch_serve.h -- host-side launchers of the serving-pool kernels.
+ * launch_serve.h -- host-side launchers of the serving pool's epoch kernels
+ * (kernels_serve.hip).
  */
 #pragma once
 
@@ -9,10 +10,13 @@
 
 namespace gray {
 
-hipError_t launch_srv_init(const SrvRequest *reqs, const SrvSlots &S, const int *a_slot,
-                           const int *a_req, const int *a_ray, int n, hipStream_t st);
-/* which: 0 FP32 Kerr, 1 FP64 Kerr, 2 FP32 grid */
-hipError_t launch_srv_epoch(int which, const SrvRequest *reqs, const SrvSlots &S, const int *list,
-                            int n, int K, const SrvLists &out, hipStream_t st);
+hipError_t launch_srv_clear(SrvCtr *ctr, hipStream_t st);
+hipError_t launch_srv_kerr(const SrvKerrArgs &A, int n_blocks, hipStream_t st);
+hipError_t launch_srv_fb64(const SrvFbArgs &A, hipStream_t st);
+hipError_t launch_srv_grid(const SrvGridArgs &A, int n_blocks, hipStream_t st);
+/* Resident blocks per CU of the Kerr (256 threads) and grid (4 waves)
+ * kernels. */
+hipError_t srv_kerr_blocks_per_cu(int *blocks);
+hipError_t srv_grid_blocks_per_cu(int *blocks);
 
 } // namespace gray

@@ -1,59 +1,146 @@
 /*
- * serve.h -- device structures of the serving pool: the request table, the
- * slot arrays (one ray per slot, SoA) and the per-epoch output lists.
+ * serve.h -- device structures of the serving pool.
+ *
+ * Requests live in a device table.  Each epoch runs three persistent
+ * kernels side by side, each with its own work queue and a bounded number
+ * of iterations per wave:
+ *   analytic Kerr, FP32 form  one lane per ray; lanes refill from their
+ *                             class's queue (rays resumed from the previous
+ *                             epoch first, then fresh rays of the class's
+ *                             requests) as rays finish;
+ *   gridded, FP32 form        one wave per 8x8-ray tile of one request,
+ *                             with its LDS brick; waves claim tiles (resumed
+ *                             tiles first, then fresh ones);
+ *   FP64 re-integration       one lane per ray handed over by the Kerr
+ *                             kernel (collapsed step or non-finite state).
+ * A ray or tile still running when its wave's iterations are used up is
+ * saved to the next epoch's resume list, state bits intact.
+ *
+ * Kerr requests are grouped in classes that share every parameter except
+ * the spin and the camera (channels, step schedule, fluid, escape rule), so
+ * that a block of the Kerr kernel serves one class and holds those in scalar
+ * registers; spin-derived values are per lane.
  */
 #pragma once
 
 #include "grid_device.h"
 #include "grayhip_params.h"
+#include "rt_thermal.h"
+#include "srv_consts.h"
 
 namespace gray {
 
-enum { SRV_KERR = 0, SRV_GRID = 1 };
-enum { SRV_NF = 4 };                  /* transfer channels carried per Kerr slot */
-enum { SRV_NC = 8 + 2 * SRV_NF };     /* components of a Kerr slot state         */
+enum { SRV_NC = 8 + 2 * SRV_NF };     /* reals of a stored (FP64) state          */
+enum { SRV_NK = 8 + SRV_NF + 1 };     /* FP32 Kerr state: geodesic, I, shared T  */
+enum { SRV_NG = 8 + 2 * SRV_NF };     /* FP32 grid state: geodesic, I/K, tau/K   */
 
 struct SrvRequest {
-	RayParams p;          /* physics and camera (n_freq <= SRV_NF)          */
-	StepRule sr;          /* per-request schedule; n_dump intervals         */
-	GridView<float> gv;   /* gridded spacetime (kind == SRV_GRID)           */
-	int kind;
-	int nd;               /* 8 + 2 n_freq: components in the output         */
-	int pad[2];
-	double *out_data;     /* [n_rays][nd]                                   */
-	double *out_info;     /* [n_rays]                                       */
-	int64_t *out_steps;   /* [n_rays]                                       */
+	RayParams p;          /* physics and camera; nu repeats channel 0 beyond
+	                         n_freq                                          */
+	StepRule sr;          /* schedule; n_dump intervals                      */
+	GridView<float> gv;   /* gridded spacetime (kind == SRV_GRID)            */
+	RtThermal rt;         /* gridded transfer constants (nu as p.nu)         */
+	int kind, nd, cls, tw;/* nd = 8 + 2 n_freq; Kerr class; tiles per row    */
+	double *out_data;     /* [n_rays][nd]                                    */
+	double *out_info;     /* [n_rays]                                        */
+	int64_t *out_steps;   /* [n_rays]                                        */
 };
 
-/* Slot arrays of capacity S.  st holds hi (rows 0..SRV_NC-1) and lo (rows
- * SRV_NC..2 SRV_NC-1) of the FP32 state, st64 the FP64 state of slots in
- * re-integration; h < 0 there asks for a restart from the camera. */
-struct SrvSlots {
-	int *req, *ray, *d, *h;
+/* A Kerr class and its share of this epoch's Kerr launch. */
+struct SrvClass {
+	StepRule sr;
+	double nu[SRV_NF];    /* channel frequencies, channel 0 repeated         */
+	double r_esc;
+	int n_freq, fluid, escape, pad;
+	int block0, block1;   /* blocks [block0, block1) serve this class        */
+	int res_off, n_resume;/* resume entries [res_off, res_off + n_resume)    */
+	int seg0, seg1;       /* fresh segments [seg0, seg1); seg_start[seg1] is
+	                         the class's fresh ray count                     */
+	unsigned items;       /* n_resume + fresh rays                           */
+};
+
+/* Rays saved at the end of a Kerr epoch (SoA, capacity cap). */
+struct SrvResume {
+	int4 *meta;           /* tab, ray, d, h                                  */
 	int64_t *steps;
-	float *st;
-	double *st64;
-	int S;
+	float *st;            /* [2 SRV_NK][cap]: hi rows, then lo rows          */
+	int cap;
 };
 
-struct SrvCounters {
-	unsigned n_fin;                  /* slots finished this epoch            */
-	unsigned n_to64;                 /* slots moved to FP64 this epoch        */
-	unsigned long long steps[3];     /* RK4 steps: FP32 Kerr, FP64 Kerr, grid */
+/* FP64 re-integration entries (SoA, capacity cap). */
+struct SrvFb {
+	int4 *meta;           /* tab, ray, d, h; h < 0: restart from the camera  */
+	int64_t *steps;
+	double *x;            /* [SRV_NC][cap]: geodesic, SRV_NF intensities, the
+	                         channels' shared depth T (tau_i = d_i T)        */
+	int cap;
 };
 
-struct SrvLists {
-	int *fin;                        /* finished slot indices                */
-	int *to64;                       /* slots moved to FP64                  */
-	SrvCounters *ctr;
+/* Grid tiles saved at the end of an epoch: record r, lane l at r 64 + l. */
+struct SrvTiles {
+	int2 *hdr;            /* per record: tab, tile                           */
+	int4 *brick;          /* per record: the wave's brick origin and cooldown
+	                         (reloaded on resume, so that every lane reads its
+	                         corners where it would have without the pause) */
+	int4 *meta;           /* per lane: ray (-1: none or finished), d, h      */
+	int64_t *steps;       /* per lane                                        */
+	float *st;            /* [2 SRV_NG][64 cap]                              */
+	int cap;              /* records                                         */
 };
 
-/* Per-lane parameters of the FP32 Kerr path (the fields getdt and the
- * transfer RHS use). */
-struct LaneKerr {
-	float a, r_h, eps_stop, r_esc, r_isco;
-	int escape, fluid, n_freq;
-	float nu[SRV_NF];
+/* Counters of one epoch (cleared by a kernel before it). */
+struct SrvCtr {
+	unsigned claimed[SRV_MAX_CLASSES];  /* Kerr queue claims per class     */
+	unsigned saved[SRV_MAX_CLASSES];    /* Kerr rays saved per class        */
+	unsigned fb_out;                    /* entries in the FP64 out list     */
+	unsigned g_claimed, g_saved;        /* grid tile claims, records saved  */
+	unsigned n_done;                    /* requests finished this epoch     */
+	unsigned long long steps[3];        /* FP32 Kerr, FP64 Kerr, grid       */
+};
+
+/* Fresh work: segment j covers queue items [start[j], start[j+1]) of its
+ * class (Kerr: rays first0 + k of request tab; grid: tiles). */
+struct SrvSegs {
+	const int *tab;
+	const int *first;
+	const unsigned *start;
+};
+
+struct SrvKerrArgs {
+	const SrvRequest *reqs;
+	const SrvClass *cls;
+	int n_cls;
+	SrvSegs seg;
+	SrvResume in, out;
+	SrvFb fb;             /* hand-overs are appended here                   */
+	SrvCtr *ctr;
+	unsigned *remaining;  /* per table index: rays not yet finished        */
+	int *done;            /* table indices of finished requests             */
+	int K;                /* iterations per wave                            */
+};
+
+struct SrvGridArgs {
+	const SrvRequest *reqs;
+	SrvSegs seg;
+	int seg0, seg1;       /* fresh tile segments                           */
+	int n_resume;         /* resumed tile records (first items)            */
+	unsigned items;       /* n_resume + fresh tiles                        */
+	SrvTiles in, out;
+	SrvCtr *ctr;
+	unsigned *remaining;
+	int *done;
+	int K;
+};
+
+struct SrvFbArgs {
+	const SrvRequest *reqs;
+	SrvFb in;
+	int n_in;
+	SrvFb out;            /* survivors are appended here                    */
+	SrvCtr *ctr;
+	unsigned *remaining;
+	int *done;
+	int K;
 };
 
 } // namespace gray

@@ -401,9 +401,15 @@ def kerr_request(side, a_spin, i_obs, nu=(), j_obs=0.0, w_img=64.0, r_obs=1000.0
 
 
 def grid_request(side, spacetime, i_obs, w_img=32.0, r_obs=35.0, n_dump=4, t_init=1.0,
-                 dt=-32.0, n_sub=1024):
+                 dt=-32.0, n_sub=1024, nu=(), h_side=None):
+    """A gridded request; channels nu use the reference's thermal flow on
+    the spacetime's density (it must be uploaded)."""
     q = Request()
-    q.w_rays = q.h_rays = side
+    q.w_rays, q.h_rays = side, h_side or side
+    q.n_freq = len(nu)
+    for i, v in enumerate(nu):
+        q.nu[i] = v
+    q.fluid = FLUID_THERMAL if nu else FLUID_NONE
     q.escape = ESC_OUTWARD
     q.w_img = q.h_img = w_img
     q.r_obs, q.i_obs = r_obs, i_obs

@@ -83,6 +83,8 @@ static void fill_request(const gray_serve_item *it, grayhip_spacetime *const gri
                          grayhip_request *q)
 {
 	static const double nus[4] = {1.0, 2.67, 4.0, 8.0};
+	/* radio to sub-mm frequencies (Hz) for the thermal emission model */
+	static const double grid_nus[4] = {2.3e11, 3.45e11, 5.0e11, 8.6e11};
 	memset(q, 0, sizeof(*q));
 	q->w_rays = q->h_rays = (size_t)it->side;
 	q->i_obs = it->i_obs;
@@ -90,12 +92,17 @@ static void fill_request(const gray_serve_item *it, grayhip_spacetime *const gri
 	q->n_sub = 1024;
 	q->dt = -32.0;
 	if (it->grid >= 0) {
-		/* camera at r = 35 M inside the 80 M box, 4 dumps of -32 */
+		/* camera at r = 35 M inside the 80 M box, 4 dumps of -32; the
+		 * channels through the reference flow on the grid's density */
 		q->spacetime = grids[it->grid];
 		q->w_img = q->h_img = 32.0;
 		q->r_obs = 35.0;
 		q->n_dump = 4;
 		q->t_init = 1.0;
+		q->n_freq = it->n_freq;
+		for (int i = 0; i < it->n_freq; ++i)
+			q->nu[i] = grid_nus[i];
+		q->fluid = GRAYHIP_FLUID_THERMAL;
 	} else {
 		/* camera at r = 1000 M, synthetic disk, 128 dumps of -32 */
 		q->a_spin = it->a_spin;
@@ -115,9 +122,11 @@ static int make_grid(int n, int device, grayhip_spacetime **out)
 	const size_t np = (size_t)n * n * n;
 	double *xs = malloc((size_t)n * sizeof(double));
 	double *gam = malloc(np * 40 * sizeof(double)), *met = malloc(np * 10 * sizeof(double));
+	double *rho = malloc(np * sizeof(double));
 	float *g32 = malloc(np * 40 * sizeof(float)), *m32 = malloc(np * 10 * sizeof(float));
+	float *r32 = malloc(np * sizeof(float));
 	int rc = -1;
-	if (!xs || !gam || !met || !g32 || !m32)
+	if (!xs || !gam || !met || !rho || !g32 || !m32 || !r32)
 		goto done;
 	gray_axis_uniform(-40.0, 40.0, n, xs);
 	grayhip_grid_desc d;
@@ -136,12 +145,15 @@ static int make_grid(int n, int device, grayhip_spacetime **out)
 	const double ax[3] = {1.0, 0.0, 0.0};
 	gray_bks_setup(&b, 1.0, 0.7, 0.3, ax);
 	for (int l = 0; l < 2; ++l) {
-		gray_bks_sample(&b, (double)l, n, n, n, xs, xs, xs, gam, met, NULL);
+		gray_bks_sample(&b, (double)l, n, n, n, xs, xs, xs, gam, met, rho);
 		for (size_t i = 0; i < np * 40; ++i)
 			g32[i] = (float)gam[i];
 		for (size_t i = 0; i < np * 10; ++i)
 			m32[i] = (float)met[i];
-		if (grayhip_spacetime_upload(*out, l, (double)l, g32, m32))
+		for (size_t i = 0; i < np; ++i)
+			r32[i] = (float)rho[i];
+		if (grayhip_spacetime_upload(*out, l, (double)l, g32, m32) ||
+		    grayhip_spacetime_upload_rho(*out, l, r32))
 			goto done;
 	}
 	rc = grayhip_spacetime_set_frozen(*out, 0);
@@ -149,8 +161,10 @@ done:
 	free(xs);
 	free(gam);
 	free(met);
+	free(rho);
 	free(g32);
 	free(m32);
+	free(r32);
 	return rc;
 }
 

@@ -52,9 +52,12 @@ class Item(C.Structure):
 
 _set = False
 
-# The request loop gives gridded requests no transfer channels (the pool
-# carries channels on analytic-Kerr requests only).
-GRID_CHANNELS = False
+# Gridded requests carry their drawn channels (the reference's thermal
+# flow on the grid's density) as analytic ones do.
+GRID_CHANNELS = True
+# Channel frequencies of the request loop (gray_serve.c fill_request).
+KERR_NUS = (1.0, 2.67, 4.0, 8.0)
+GRID_NUS = (2.3e11, 3.45e11, 5.0e11, 8.6e11)
 
 
 def served_channels(item):

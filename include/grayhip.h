@@ -275,6 +275,7 @@ int  grayhip_server_result(grayhip_server *srv, int64_t id, double *data, double
                            int64_t *steps);
 /* Totals since creation: RK4 steps (FP32 Kerr, FP64 Kerr, gridded), epochs. */
 int  grayhip_server_stats(grayhip_server *srv, int64_t steps[3], int64_t *epochs);
+/* Message of the calling thread's last failed grayhip_server_* call. */
 const char *grayhip_server_error(const grayhip_server *srv);
 
 /* ---- a render that allocates nothing ----
