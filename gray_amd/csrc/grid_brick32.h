@@ -76,7 +76,7 @@ template <class S>
 __device__ __forceinline__ void brick_store(float4 *brick, int p, const S *src)
 {
 	float v[40];
-	load40(src, v);
+	load40(gmem(src), v);
 #pragma unroll
 	for (int i = 0; i < 10; ++i)
 		brick[p * 10 + i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
@@ -87,7 +87,7 @@ __device__ __forceinline__ void brick_store(float4 *brick, int p, const S *src)
 template <int NF, class S>
 __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThermalF &rt,
                                               float4 *brick, BrickHdr *hdr, bool use_lds,
-                                              const float *s, float *ds)
+                                              const float *s, float *ds, int nf = NF)
 {
 	Stencil<float> st;
 	make_stencil<float, S>(G, s[0], s[1], s[2], s[3], st);
@@ -161,15 +161,21 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 	for (int mu = 0; mu < 4; ++mu)
 		ds[4 + mu] = -uGu[mu] + uGu[0] * u[mu];
 	if constexpr (NF > 0) {
-		float kt, ne;
-		lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
-		if (st.wt > 0.f) {
-			float kt2, ne2;
-			lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
-			kt = kt + st.wt * (kt2 - kt);
-			ne = ne + st.wt * (ne2 - ne);
+		if (nf > 0) {
+			float kt, ne;
+			lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
+			if (st.wt > 0.f) {
+				float kt2, ne2;
+				lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
+				kt = kt + st.wt * (kt2 - kt);
+				ne = ne + st.wt * (ne2 - ne);
+			}
+			thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF, nf);
+		} else {
+#pragma unroll
+			for (int i = 8; i < 8 + 2 * NF; ++i)
+				ds[i] = 0.f;
 		}
-		thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
 	}
 }
 
@@ -177,30 +183,30 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 template <int NF, class S>
 __device__ __forceinline__ void grid_rk4_brick(const GridView<S> &G, const RtThermalF &rt,
                                               float4 *brick, BrickHdr *hdr, bool use_lds,
-                                              float *hi, float *lo, float dt)
+                                              float *hi, float *lo, float dt, int nf = NF)
 {
 	constexpr int ND = 8 + 2 * NF;
 	float k[ND], acc[ND], x[ND];
 	const float hdt = 0.5f * dt;
-	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, hi, k);
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, hi, k, nf);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] = k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k, nf);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
-	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k, nf);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + dt * k[i];
 	}
-	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k);
+	grid_rhs_brick<NF, S>(G, rt, brick, hdr, use_lds, x, k, nf);
 	const float dt6 = dt * (1.0f / 6.0f);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {

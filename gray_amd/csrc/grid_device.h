@@ -22,6 +22,19 @@
 
 namespace gray {
 
+/* Global-memory pointers.  The grid's level pointers reach the serving
+ * kernels through a request table in memory, where the compiler cannot
+ * tell their address space and would read through flat loads (which also
+ * count against LDS waits); the grid readers load through these instead. */
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+
+template <class T>
+__device__ __forceinline__ gptr<T> gmem(const T *p)
+{
+	return (gptr<T>)p;
+}
+
 template <class S>
 struct GridView {
 	const S *gam[2];      /* per level: [p][40]                        */

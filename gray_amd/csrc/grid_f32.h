@@ -12,13 +12,17 @@
 
 namespace gray {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 /* 40 values of one grid point as floats, read with 16-byte loads. */
-__device__ __forceinline__ void load40(const float *__restrict__ a, float v[40])
+__device__ __forceinline__ void load40(gptr<float> a, float v[40])
 {
-	const float4 *a4 = reinterpret_cast<const float4 *>(a);
+	gptr<f32x4> a4 = (gptr<f32x4>)a;
 #pragma unroll
 	for (int i = 0; i < 10; ++i) {
-		float4 t = a4[i];
+		const f32x4 t = a4[i];
 		v[4 * i] = t.x;
 		v[4 * i + 1] = t.y;
 		v[4 * i + 2] = t.z;
@@ -26,12 +30,12 @@ __device__ __forceinline__ void load40(const float *__restrict__ a, float v[40])
 	}
 }
 
-__device__ __forceinline__ void load40(const double *__restrict__ a, float v[40])
+__device__ __forceinline__ void load40(gptr<double> a, float v[40])
 {
-	const double2 *a2 = reinterpret_cast<const double2 *>(a);
+	gptr<f64x2> a2 = (gptr<f64x2>)a;
 #pragma unroll
 	for (int i = 0; i < 20; ++i) {
-		double2 t = a2[i];
+		const f64x2 t = a2[i];
 		v[2 * i] = (float)t.x;
 		v[2 * i + 1] = (float)t.y;
 	}
@@ -39,12 +43,12 @@ __device__ __forceinline__ void load40(const double *__restrict__ a, float v[40]
 
 /* Half a point's Christoffels (mu = 0, 1 or mu = 2, 3): a corner is read
  * in two halves so that only 20 values are live at a time. */
-__device__ __forceinline__ void load20(const float *__restrict__ a, float v[20])
+__device__ __forceinline__ void load20(gptr<float> a, float v[20])
 {
-	const float4 *a4 = reinterpret_cast<const float4 *>(a);
+	gptr<f32x4> a4 = (gptr<f32x4>)a;
 #pragma unroll
 	for (int i = 0; i < 5; ++i) {
-		float4 t = a4[i];
+		const f32x4 t = a4[i];
 		v[4 * i] = t.x;
 		v[4 * i + 1] = t.y;
 		v[4 * i + 2] = t.z;
@@ -52,12 +56,12 @@ __device__ __forceinline__ void load20(const float *__restrict__ a, float v[20])
 	}
 }
 
-__device__ __forceinline__ void load20(const double *__restrict__ a, float v[20])
+__device__ __forceinline__ void load20(gptr<double> a, float v[20])
 {
-	const double2 *a2 = reinterpret_cast<const double2 *>(a);
+	gptr<f64x2> a2 = (gptr<f64x2>)a;
 #pragma unroll
 	for (int i = 0; i < 10; ++i) {
-		double2 t = a2[i];
+		const f64x2 t = a2[i];
 		v[2 * i] = (float)t.x;
 		v[2 * i + 1] = (float)t.y;
 	}
@@ -77,7 +81,7 @@ __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const St
 #pragma unroll
 		for (int hf = 0; hf < 2; ++hf) {
 			float v[20];
-			load20(lev + (size_t)st.corner(k) * 40 + 20 * hf, v);
+			load20(gmem(lev) + (size_t)st.corner(k) * 40 + 20 * hf, v);
 #pragma unroll
 			for (int m2 = 0; m2 < 2; ++m2) {
 				float t = 0.f;
@@ -92,17 +96,17 @@ __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const St
 
 /* sum_k w_k (g_{t nu}(corner k) u^nu) and sum_k w_k rho(corner k): k_t and
  * n_e of one level (interpolation commutes with the contraction). */
-__device__ __forceinline__ void load4(const float *__restrict__ a, float v[4])
+__device__ __forceinline__ void load4(gptr<float> a, float v[4])
 {
-	const float2 *a2 = reinterpret_cast<const float2 *>(a);   /* p*40 B: 8-byte aligned */
-	const float2 x = a2[0], y = a2[1];
+	gptr<f32x2> a2 = (gptr<f32x2>)a;   /* p*40 B: 8-byte aligned */
+	const f32x2 x = a2[0], y = a2[1];
 	v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
 }
 
-__device__ __forceinline__ void load4(const double *__restrict__ a, float v[4])
+__device__ __forceinline__ void load4(gptr<double> a, float v[4])
 {
-	const double2 *a2 = reinterpret_cast<const double2 *>(a);
-	const double2 x = a2[0], y = a2[1];
+	gptr<f64x2> a2 = (gptr<f64x2>)a;
+	const f64x2 x = a2[0], y = a2[1];
 	v[0] = (float)x.x; v[1] = (float)x.y; v[2] = (float)y.x; v[3] = (float)y.y;
 }
 
@@ -117,10 +121,10 @@ __device__ __forceinline__ void lev_flow(const S *__restrict__ met, const S *__r
 	for (int k = 0; k < 8; ++k) {
 		const uint32_t p = st.corner(k);
 		float m[4];
-		load4(met + (size_t)p * 10, m);
+		load4(gmem(met) + (size_t)p * 10, m);
 		const float w = st.weight(k);
 		kt += w * (((m[0] * u[0] + m[1] * u[1]) + m[2] * u[2]) + m[3] * u[3]);
-		ne += w * (float)rho[p];
+		ne += w * (float)gmem(rho)[p];
 	}
 }
 

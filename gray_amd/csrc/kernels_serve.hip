@@ -80,7 +80,9 @@ __device__ __forceinline__ void seg_item(const SrvSegs &S, int s0, int s1, unsig
 /* ---------------- analytic Kerr, FP32 form ---------------- */
 
 template <int NF>
-__device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A, const SrvClass &C, int c)
+__device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
+                                           const SrvRequest *__restrict__ reqs,
+                                           const SrvClass &C, int c)
 {
 	constexpr int ND = nd32<NF>();
 	const unsigned lane = __lane_id();
@@ -130,14 +132,14 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A, const SrvClass 
 				} else {
 					seg_item(A.seg, C.seg0, C.seg1, idx - (unsigned)n_resume, tab, ray);
 					double x[8 + 2 * NF];
-					icond_ray<NF, RayParams>(A.reqs[tab].p, (size_t)ray, x);
+					icond_ray<NF, RayParams>(reqs[tab].p, (size_t)ray, x);
 					split_state<NF, SrvClass>(C, x, hi, lo);
 					d = 0;
 					h = 0;
 					st = 0;
 				}
 				if (tab >= 0) {
-					const RayParams &p = A.reqs[tab].p;
+					const RayParams &p = reqs[tab].p;
 					P.a = (float)p.a;
 					P.r_h = (float)p.r_h;
 					P.eps_stop = (float)p.eps_stop;
@@ -194,7 +196,7 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A, const SrvClass 
 		if (state == 0)
 			continue;
 		atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
-		const SrvRequest &R = A.reqs[tab];
+		const SrvRequest &R = reqs[tab];
 		if (state == 1) {
 			double x[8 + 2 * NF];
 			join_state<NF, SrvClass>(C, hi, lo, x);
@@ -228,19 +230,23 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A, const SrvClass 
 	}
 }
 
-__global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A)
+/* The request and class tables come as restrict parameters besides A: the
+ * kernels write only through other pointers, so their loads stay scalar. */
+__global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A,
+                                                        const SrvRequest *__restrict__ reqs,
+                                                        const SrvClass *__restrict__ cls)
 {
 	const int b = (int)blockIdx.x;
 	for (int c = 0; c < A.n_cls; ++c) {
-		const SrvClass &C = A.cls[c];
+		const SrvClass &C = cls[c];
 		if (b < C.block0 || b >= C.block1)
 			continue;
 		switch (C.n_freq) {
-		case 0: kerr_lanes<0>(A, C, c); break;
-		case 1: kerr_lanes<1>(A, C, c); break;
-		case 2: kerr_lanes<2>(A, C, c); break;
-		case 3: kerr_lanes<3>(A, C, c); break;
-		default: kerr_lanes<4>(A, C, c); break;
+		case 0: kerr_lanes<0>(A, reqs, C, c); break;
+		case 1: kerr_lanes<1>(A, reqs, C, c); break;
+		case 2: kerr_lanes<2>(A, reqs, C, c); break;
+		case 3: kerr_lanes<3>(A, reqs, C, c); break;
+		default: kerr_lanes<4>(A, reqs, C, c); break;
 		}
 		return;
 	}
@@ -248,7 +254,8 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A)
 
 /* ---------------- FP64 re-integration ---------------- */
 
-__global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A)
+__global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A,
+                                                        const SrvRequest *__restrict__ reqs)
 {
 	const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (e >= A.n_in)
@@ -256,7 +263,7 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A)
 	const int4 mt = A.in.meta[e];
 	const int tab = mt.x, ray = mt.y;
 	int d = mt.z, h = mt.w;
-	const SrvRequest &R = A.reqs[tab];
+	const SrvRequest &R = reqs[tab];
 	const StepRule sr = R.sr;
 	const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
 	double x[SRV_NC];
@@ -355,15 +362,18 @@ __device__ __forceinline__ void grid_icond_ray(const SrvRequest &R, int g, doubl
 }
 
 /* One tile of a gridded request from its current state until its rays are
- * done or the wave's iterations run out; then saves the live lanes.
- * Returns the iterations used. */
+ * done, the wave's iterations (it counts them) run out or the launch's stop
+ * time passes; then saves the live lanes.  Returns whether it saved. */
 template <int NF>
-__device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile, bool resumed,
-                                         int rec, float4 *brick, BrickHdr *hdr, int it)
+__device__ __forceinline__ bool grid_tile(const SrvGridArgs &A, const SrvRequest *__restrict__ reqs,
+                                          int tab, int tile, bool resumed, int rec, float4 *brick,
+                                          BrickHdr *hdr, int &it, unsigned long long end, int nf)
 {
 	constexpr int ND = 8 + 2 * NF;
 	const unsigned lane = __lane_id();
-	const SrvRequest &R = A.reqs[tab];
+	const SrvRequest &R = reqs[tab];
+	/* the spacetime read through the table (uniform loads): a copy would
+	 * hold ~70 more registers and halve the kernel's occupancy */
 	const GridView<float> &G = R.gv;
 	const StepRule sr = R.sr;
 	const RtThermalF rt = rt_thermal_f32(R.rt);
@@ -427,7 +437,7 @@ __device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile
 	const int64_t st0 = st;
 	bool live = ray >= 0;
 	for (; it < A.K; ++it) {
-		if (!__ballot(live))
+		if (!__ballot(live) || (unsigned long long)wall_clock64() >= end)
 			break;
 		if (live) {
 			const float gdt = grid_getdt<float, float>(G, hi[1], hi[2], hi[3], hi[5], hi[6],
@@ -439,7 +449,7 @@ __device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile
 				if (dh > sr.n - h)
 					dh = sr.n - h;
 				grid_rk4_brick<NF, float>(G, rt, brick, hdr, use_lds, hi, lo,
-				                          (float)((double)dh * dt_over_n));
+				                          (float)((double)dh * dt_over_n), nf);
 				++st;
 				h += dh;
 				if (h >= sr.n) {
@@ -463,8 +473,9 @@ __device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile
 	}
 	atomicAdd(&A.ctr->steps[2], (unsigned long long)(st - st0));
 	wave_sync();   /* the brick header as the last evaluation left it */
-	if (__ballot(live)) {
-		/* iterations used up: the tile goes on next epoch */
+	const bool save = __ballot(live) != 0;
+	if (save) {
+		/* iterations or time used up: the tile goes on next epoch */
 		unsigned r = 0;
 		if (lane == 0)
 			r = atomicAdd(&A.ctr->g_saved, 1u);
@@ -484,10 +495,35 @@ __device__ __forceinline__ int grid_tile(const SrvGridArgs &A, int tab, int tile
 			}
 		}
 	}
-	return it;
+	return save;
 }
 
-__global__ __launch_bounds__(64 * SRV_GWAVES) void k_srv_grid(SrvGridArgs A)
+/* The launch's stop time: the first wave to start sets it to its start plus
+ * the budget, so that every wave stops at about the same moment and none
+ * idles while another finishes a fixed count of slower iterations. */
+__device__ __forceinline__ unsigned long long grid_stop_time(const SrvGridArgs &A)
+{
+	if (A.budget == 0)
+		return ~0ull;
+	unsigned long long end = 0;
+	if (__lane_id() == 0) {
+		const unsigned long long want = (unsigned long long)wall_clock64() + A.budget;
+		const unsigned long long old = atomicCAS(&A.ctr->g_end, 0ull, want);
+		end = old ? old : want;
+	}
+	const unsigned lo = __builtin_amdgcn_readfirstlane(__shfl((unsigned)end, 0));
+	const unsigned hi = __builtin_amdgcn_readfirstlane(__shfl((unsigned)(end >> 32), 0));
+	return ((unsigned long long)hi << 32) | lo;
+}
+
+/* Every gridded request's tiles, built for SRV_NF channels and run with the
+ * request's count (uniform per wave; the others cost no transcendentals):
+ * two waves per SIMD, as the LDS bricks allow.  A wave claims tiles until
+ * the queue is empty or its current tile is saved; it claims at least once,
+ * so the resumed records (the first items, no more than the waves) are all
+ * taken. */
+__global__ __launch_bounds__(64 * SRV_GWAVES) __attribute__((amdgpu_waves_per_eu(2)))
+void k_srv_grid(SrvGridArgs A, const SrvRequest *__restrict__ reqs)
 {
 	__shared__ float4 s_brick[SRV_GWAVES][GP_PTS * 10];
 	__shared__ BrickHdr s_hdr[SRV_GWAVES];
@@ -496,7 +532,8 @@ __global__ __launch_bounds__(64 * SRV_GWAVES) void k_srv_grid(SrvGridArgs A)
 	float4 *brick = s_brick[wv];
 	BrickHdr *hdr = &s_hdr[wv];
 	int it = 0;
-	while (it < A.K) {
+	const unsigned long long end = grid_stop_time(A);
+	for (;;) {
 		unsigned item = 0;
 		if (lane == 0)
 			item = atomicAdd(&A.ctr->g_claimed, 1u);
@@ -515,13 +552,10 @@ __global__ __launch_bounds__(64 * SRV_GWAVES) void k_srv_grid(SrvGridArgs A)
 		}
 		tab = __builtin_amdgcn_readfirstlane(tab);
 		tile = __builtin_amdgcn_readfirstlane(tile);
-		switch (A.reqs[tab].p.n_freq) {
-		case 0: it = grid_tile<0>(A, tab, tile, resumed, rec, brick, hdr, it); break;
-		case 1: it = grid_tile<1>(A, tab, tile, resumed, rec, brick, hdr, it); break;
-		case 2: it = grid_tile<2>(A, tab, tile, resumed, rec, brick, hdr, it); break;
-		case 3: it = grid_tile<3>(A, tab, tile, resumed, rec, brick, hdr, it); break;
-		default: it = grid_tile<4>(A, tab, tile, resumed, rec, brick, hdr, it); break;
-		}
+		const int nf = __builtin_amdgcn_readfirstlane(reqs[tab].p.n_freq);
+		if (grid_tile<SRV_NF>(A, reqs, tab, tile, resumed, rec, brick, hdr, it, end, nf) ||
+		    it >= A.K || (unsigned long long)wall_clock64() >= end)
+			break;
 	}
 }
 
@@ -545,7 +579,7 @@ hipError_t launch_srv_kerr(const SrvKerrArgs &A, int n_blocks, hipStream_t st)
 {
 	if (n_blocks <= 0 || A.n_cls <= 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(k_srv_kerr, dim3(n_blocks), dim3(SRV_BLOCK), 0, st, A);
+	hipLaunchKernelGGL(k_srv_kerr, dim3(n_blocks), dim3(SRV_BLOCK), 0, st, A, A.reqs, A.cls);
 	return hipGetLastError();
 }
 
@@ -554,7 +588,7 @@ hipError_t launch_srv_fb64(const SrvFbArgs &A, hipStream_t st)
 	if (A.n_in <= 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(k_srv_fb64, dim3(blocks_for((size_t)A.n_in, SRV_BLOCK)), dim3(SRV_BLOCK),
-	                   0, st, A);
+	                   0, st, A, A.reqs);
 	return hipGetLastError();
 }
 
@@ -562,7 +596,7 @@ hipError_t launch_srv_grid(const SrvGridArgs &A, int n_blocks, hipStream_t st)
 {
 	if (n_blocks <= 0 || A.items == 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(k_srv_grid, dim3(n_blocks), dim3(64 * SRV_GWAVES), 0, st, A);
+	hipLaunchKernelGGL(k_srv_grid, dim3(n_blocks), dim3(64 * SRV_GWAVES), 0, st, A, A.reqs);
 	return hipGetLastError();
 }
 

@@ -119,14 +119,23 @@ __host__ __device__ inline RtThermalF rt_thermal_f32(const RtThermal &c)
 	return f;
 }
 
+/* Channels i < nf of NF (nf uniform across the wave, as the serving pool
+ * runs requests with fewer channels in a kernel built for more); the
+ * others get zero derivatives. */
 template <int NF>
 __device__ __forceinline__ void thermal_f32(const RtThermalF &c, float ne, float shift,
-                                            const float *tau_s, float *dI_s, float *dtau_s)
+                                            const float *tau_s, float *dI_s, float *dtau_s,
+                                            int nf = NF)
 {
 	const float x = c.ff_x_te;
 	const float inv_sh2 = 1.0f / (shift * shift + 1e-28f);
 #pragma unroll
 	for (int i = 0; i < NF; ++i) {
+		if (i >= nf) {
+			dI_s[i] = 0.f;
+			dtau_s[i] = 0.f;
+			continue;
+		}
 		const float nu = c.nu[i] * shift;
 		/* B_nu at T_e (rt.cl:80-93) */
 		const float n = nu * c.inv_c;

@@ -96,6 +96,8 @@ struct SrvCtr {
 	unsigned g_claimed, g_saved;        /* grid tile claims, records saved  */
 	unsigned n_done;                    /* requests finished this epoch     */
 	unsigned long long steps[3];        /* FP32 Kerr, FP64 Kerr, grid       */
+	unsigned long long g_end;           /* grid launch's stop time (wall
+	                                       clock ticks; 0 until set)        */
 };
 
 /* Fresh work: segment j covers queue items [start[j], start[j+1]) of its
@@ -130,6 +132,8 @@ struct SrvGridArgs {
 	unsigned *remaining;
 	int *done;
 	int K;
+	unsigned long long budget;   /* wall clock ticks from the first wave's start
+	                                to the stop (0: K iterations only)    */
 };
 
 struct SrvFbArgs {

@@ -12,12 +12,18 @@ namespace srv {
 
 namespace {
 
-/* Relative cost of a grid tile against a Kerr ray when the GPU is shared
- * out between the two launches: a tile is 64 rays of ~1e4 steps that cost
- * ~3.6 times a Kerr step each, against ~6.4e4 steps per Kerr ray. */
-constexpr double GRID_TILE_COST = 64.0 * 1e4 * 3.6 / 6.4e4;
+/* Priors of the work model that shares the GPU between the Kerr and grid
+ * launches, until measurements take over: steps per unit (a Kerr ray; a
+ * grid tile of 64 rays of ~1e4 steps) and block-milliseconds per step (a
+ * lane takes ~4.4 us per Kerr step, ~20 us per gridded step with
+ * channels; 256 lanes per block). */
+constexpr double PRIOR_SPU[2] = {6.4e4, 64.0 * 1e4};
+constexpr double PRIOR_BMS[2] = {4.4e-3 / SRV_BLOCK, 2.0e-2 / SRV_BLOCK};
+constexpr double PRIOR_UNITS[2] = {4096.0, 64.0};   /* weight of the priors */
 constexpr size_t KERR_CHUNK = 256;   /* rays per request per queue round   */
 constexpr size_t GRID_CHUNK = 4;     /* tiles per request per queue round  */
+constexpr size_t GRID_FRESH = 4;     /* fresh tiles offered per grid wave  */
+constexpr int GRID_K_BOUND = 1 << 20;
 
 void set_err(char *err, size_t len, const char *msg)
 {
@@ -91,6 +97,19 @@ Sched::Sched(const Config &cfg, const Caps &caps, Device *dev)
 	K_[1] = K;
 	K_[2] = std::max(16, K / 4);
 	steps_[0] = steps_[1] = steps_[2] = 0;
+	for (int k = 0; k < 2; ++k) {
+		units_[k] = 0.0;
+		bms_[k] = PRIOR_BMS[k];
+		bms_seen_[k] = false;
+	}
+}
+
+/* Steps per unit of kind k (0 Kerr ray, 1 grid tile): the steps taken over
+ * the units started, with the prior's weight. */
+double Sched::spu(int k) const
+{
+	const double st = (double)(k == 0 ? steps_[0] + steps_[1] : steps_[2]);
+	return (st + PRIOR_SPU[k] * PRIOR_UNITS[k]) / (units_[k] + PRIOR_UNITS[k]);
 }
 
 int Sched::submit(Req r, int64_t *id, char *err, size_t errlen)
@@ -199,7 +218,12 @@ void Sched::plan_locked(Plan &p)
 	p.flip = flip_;
 	p.K_kerr = K_[0];
 	p.K_fb = K_[1];
-	p.K_grid = K_[2];
+	/* the grid launch stops on time rather than after a count of
+	 * iterations: its waves' iterations differ in cost by the cells they
+	 * read, and a count would leave the fast ones idle while the slowest
+	 * finishes (K_grid stays as a bound) */
+	p.grid_ms = cfg_.epoch_ms;
+	p.K_grid = cfg_.epoch_ms > 0 ? GRID_K_BOUND : K_[2];
 	p.n_fb = n_fb_;
 
 	/* work left per class and for the grid kind */
@@ -230,12 +254,16 @@ void Sched::plan_locked(Plan &p)
 			wk_all += wk[c];
 		}
 	}
-	wg = (wg + g_resume_) * GRID_TILE_COST;
+	const double tiles_left = wg + g_resume_;
 
-	/* the GPU shared between the two launches by their work */
+	/* the GPU shared between the two launches by the time their work left
+	 * would take on all of it: units x steps per unit x block-ms per step /
+	 * blocks the GPU holds */
 	int kerr_total = caps_.kerr_blocks, grid_total = caps_.grid_blocks;
-	if (wk_all > 0 && wg > 0) {
-		const double fg = wg / (wg + wk_all);
+	if (wk_all > 0 && tiles_left > 0) {
+		const double tk = wk_all * spu(0) * bms_[0] / caps_.kerr_blocks;
+		const double tg = tiles_left * spu(1) * bms_[1] / caps_.grid_blocks;
+		const double fg = tg / (tg + tk);
 		kerr_total = std::max(1, (int)(caps_.kerr_blocks * (1.0 - fg) + 0.5));
 		grid_total = std::max(1, (int)(caps_.grid_blocks * fg + 0.5));
 	}
@@ -304,13 +332,12 @@ void Sched::plan_locked(Plan &p)
 
 	/* grid: at least a wave per resumed tile record, no more blocks than
 	 * the tiles can fill */
-	if (wg > 0) {
-		const int tiles_left = (int)(wg / GRID_TILE_COST + 0.5);
+	if (tiles_left > 0) {
 		const int mn = std::max(1, (g_resume_ + SRV_GWAVES - 1) / SRV_GWAVES);
-		const int want = std::max(mn, (tiles_left + SRV_GWAVES - 1) / SRV_GWAVES);
+		const int want = std::max(mn, ((int)tiles_left + SRV_GWAVES - 1) / SRV_GWAVES);
 		p.grid_blocks = std::max(mn, std::min(want, grid_total));
 		p.g_resume = g_resume_;
-		size_t want_fresh = 2 * (size_t)p.grid_blocks * SRV_GWAVES;
+		size_t want_fresh = GRID_FRESH * (size_t)p.grid_blocks * SRV_GWAVES;
 		want_fresh = want_fresh > (size_t)g_resume_ ? want_fresh - (size_t)g_resume_ : 0;
 		unsigned fresh = 0;
 		fill_segs(grid_reqs, want_fresh, GRID_CHUNK, (size_t)caps_.max_segs, p.gseg, &fresh);
@@ -328,11 +355,13 @@ void Sched::consume_locked(const Plan &p, const Result &res)
 			n -= k;
 		}
 	};
+	double fresh[2] = {0.0, 0.0};
 	for (int k = 0; k < p.n_cls; ++k) {
 		const ClassPlan &cp = p.cls[k];
 		const int c = cp.slot;
 		const unsigned taken = std::min(res.claimed[k], cp.items);
 		advance(p.kseg, cp.seg0, cp.seg1, taken - (unsigned)cp.n_resume);
+		fresh[0] += taken - (unsigned)cp.n_resume;
 		cls_[c].n_resume = (int)res.saved[k];
 		cls_[c].res_off = cp.block0 * SRV_BLOCK;
 	}
@@ -347,6 +376,7 @@ void Sched::consume_locked(const Plan &p, const Result &res)
 	if (p.g_items > 0) {
 		const unsigned taken = std::min(res.g_claimed, p.g_items);
 		advance(p.gseg, 0, (int)p.gseg.size(), taken - (unsigned)p.g_resume);
+		fresh[1] += taken - (unsigned)p.g_resume;
 		g_resume_ = (int)res.g_saved;
 	} else {
 		g_resume_ = 0;
@@ -367,6 +397,22 @@ void Sched::consume_locked(const Plan &p, const Result &res)
 	for (int k = 0; k < 3; ++k)
 		steps_[k] += res.steps[k];
 	++epochs_;
+	/* work model: units started and steps taken per kind; block-ms per
+	 * step from epochs whose launch was busy throughout (it saved work) */
+	units_[0] += fresh[0];
+	units_[1] += fresh[1];
+	const double ksteps[2] = {(double)res.steps[0], (double)res.steps[2]};
+	const double kms[2] = {res.ms[0], res.ms[2]};
+	const int kblk[2] = {p.kerr_blocks, p.grid_blocks};
+	bool kfull[2] = {false, res.g_saved > 0};
+	for (int k = 0; k < p.n_cls; ++k)
+		kfull[0] = kfull[0] || res.saved[k] > 0;
+	for (int k = 0; k < 2; ++k)
+		if (kfull[k] && kms[k] > 0.0 && ksteps[k] > 0.0) {
+			const double b = kms[k] * kblk[k] / ksteps[k];
+			bms_[k] = bms_seen_[k] ? 0.7 * bms_[k] + 0.3 * b : b;
+			bms_seen_[k] = true;
+		}
 	/* iterations per wave toward the target epoch time, from kernels that
 	 * ran them all (they saved work for the next epoch) */
 	if (cfg_.epoch_ms > 0) {

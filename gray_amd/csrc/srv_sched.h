@@ -80,6 +80,7 @@ struct Plan {
 	std::vector<Seg> kseg;
 	int kerr_blocks, K_kerr;
 	int grid_blocks, K_grid;
+	double grid_ms;       /* grid launch's time budget (<= 0: K_grid alone) */
 	int g_resume;         /* resumed tile records                          */
 	std::vector<Seg> gseg;
 	unsigned g_items;
@@ -170,6 +171,11 @@ private:
 	int n_fb_;                            /* FP64 entries for the next epoch   */
 	int flip_;
 	int K_[3];                            /* Kerr, FP64, grid iterations       */
+	/* work model of the two launches (0 Kerr, 1 grid) */
+	double units_[2];                     /* units started                     */
+	double bms_[2];                       /* block-ms per step                 */
+	bool bms_seen_[2];
+	double spu(int k) const;
 	int64_t next_id_, epochs_;
 	unsigned long long steps_[3];
 };
