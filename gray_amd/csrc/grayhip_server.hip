@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdarg>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -61,6 +62,8 @@ public:
 	/* pinned staging */
 	SrvCtr *h_ctr = nullptr;
 	int *h_done = nullptr;
+	volatile unsigned *h_started = nullptr;   /* written by the GPU (coherent) */
+	unsigned epoch_no = 0;
 	SrvClass *h_cls = nullptr;
 	int *h_kseg = nullptr, *h_gseg = nullptr;
 	/* messages per calling thread: an epoch and a result copy may fail on
@@ -87,13 +90,15 @@ public:
 	{
 		cfg = c;
 		hipError_t e = hipSetDevice(c.device);
-		int n_cu = 0, kb = 0, gb = 0;
+		int n_cu = 0, kb = 0, gb = 0, kbh = 0;
 		if (e == hipSuccess)
 			e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device);
 		if (e == hipSuccess)
 			e = srv_kerr_blocks_per_cu(&kb);
 		if (e == hipSuccess)
 			e = srv_grid_blocks_per_cu(&gb);
+		if (e == hipSuccess)
+			e = srv_kerr_beside_heavy(&kbh);
 		if (e == hipSuccess)
 			e = hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c.device);
 		if (e == hipSuccess && clock_khz <= 0)
@@ -104,13 +109,19 @@ public:
 		const int want = (int)std::min<size_t>((c.slots + SRV_BLOCK - 1) / SRV_BLOCK, 1u << 20);
 		caps.kerr_blocks = std::max(1, std::min(std::max(kb, 1) * n_cu, want));
 		caps.grid_blocks = std::max(1, std::min(std::max(gb, 1) * n_cu, want));
+		caps.n_cu = std::max(1, n_cu);
+		caps.kerr_beside = std::max(0, std::min(kbh, std::max(kb, 1)));
 		caps.fb_cap = SRV_FB_CAP;
 		caps.max_segs = SRV_MAX_SEGS;
 		caps.max_requests = c.max_requests;
 		caps.arena = c.hbm_budget;
 		const size_t R = (size_t)c.max_requests;
-		const int kcap = (caps.kerr_blocks + SRV_MAX_CLASSES) * SRV_BLOCK;
-		const int gcap = caps.grid_blocks * SRV_GWAVES + SRV_GWAVES;
+		/* an epoch writes at most max(lanes, entries resumed) per class (or
+		 * waves, records for the grid): twice a launch's lanes bound it */
+		const int kcap = 2 * (caps.kerr_blocks + SRV_MAX_CLASSES) * SRV_BLOCK;
+		const int gcap = 2 * (caps.grid_blocks * SRV_GWAVES + SRV_GWAVES);
+		caps.res_cap = kcap;
+		caps.tile_cap = gcap;
 		struct { void **p; size_t b; } al[] = {
 			{(void **)&d_reqs, R * sizeof(SrvRequest)},
 			{(void **)&d_remaining, R * sizeof(unsigned)},
@@ -148,6 +159,8 @@ public:
 				e = hipMalloc(a.p, a.b);
 		if (e == hipSuccess)
 			e = hipHostMalloc((void **)&h_ctr, sizeof(SrvCtr), 0);
+		if (e == hipSuccess)
+			e = hipHostMalloc((void **)&h_started, sizeof(unsigned), hipHostMallocCoherent);
 		if (e == hipSuccess)
 			e = hipHostMalloc((void **)&h_done, R * sizeof(int), 0);
 		if (e == hipSuccess)
@@ -195,7 +208,7 @@ public:
 			(void)hipFree(tiles[k].steps);
 			(void)hipFree(tiles[k].st);
 		}
-		void *host[] = {h_ctr, h_done, h_cls, h_kseg, h_gseg};
+		void *host[] = {h_ctr, h_done, h_cls, h_kseg, h_gseg, (void *)h_started};
 		for (void *p : host)
 			(void)hipHostFree(p);
 	}
@@ -305,15 +318,22 @@ public:
 		DCK(hipSetDevice(cfg.device));
 		if (p.kseg.size() > (size_t)SRV_MAX_SEGS || p.gseg.size() > (size_t)SRV_MAX_SEGS)
 			return fail(GRAYHIP_EDEVICE, "too many queue segments");
-		/* the lists the kernels index: Kerr lanes save into [block0 256,
-		 * block1 256) of a resume list, grid waves one record each */
-		if (p.kerr_blocks > caps.kerr_blocks + SRV_MAX_CLASSES || p.grid_blocks > caps.grid_blocks ||
-		    p.n_fb > SRV_FB_CAP || p.g_resume > tiles[0].cap || p.n_cls > SRV_MAX_CLASSES)
+		/* the lists the kernels index */
+		bool ok = p.kerr_blocks <= caps.kerr_blocks + SRV_MAX_CLASSES &&
+		          p.grid_blocks <= caps.grid_blocks && p.fb_blocks <= caps.kerr_blocks &&
+		          p.n_cls <= SRV_MAX_CLASSES && p.n_fb >= 0 && p.fb_off >= 0 &&
+		          p.fb_off + p.n_fb <= fb[0].cap && p.g_resume >= 0 && p.g_res_off >= 0 &&
+		          p.g_res_off + p.g_resume <= tiles[0].cap && p.g_out_cap <= tiles[0].cap &&
+		          (p.grid_blocks == 0 || p.g_out_cap >= p.grid_blocks * SRV_GWAVES);
+		for (int k = 0; k < p.n_cls && ok; ++k) {
+			const srv::ClassPlan &cp = p.cls[k];
+			ok = cp.block0 >= 0 && cp.block0 < cp.block1 && cp.block1 <= p.kerr_blocks &&
+			     cp.res_off >= 0 && cp.res_off + cp.n_resume <= res[0].cap && cp.out_base >= 0 &&
+			     cp.out_base + cp.out_cap <= res[0].cap &&
+			     cp.out_cap >= (cp.block1 - cp.block0) * SRV_BLOCK && cp.out_cap >= cp.n_resume;
+		}
+		if (!ok)
 			return fail(GRAYHIP_EDEVICE, "epoch plan beyond the pool's lists");
-		for (int k = 0; k < p.n_cls; ++k)
-			if (p.cls[k].block1 > p.kerr_blocks ||
-			    p.cls[k].res_off + p.cls[k].n_resume > res[0].cap)
-				return fail(GRAYHIP_EDEVICE, "class plan beyond the resume lists");
 		const int in = p.flip, o = p.flip ^ 1;
 		for (int k = 0; k < p.n_cls; ++k) {
 			const srv::ClassPlan &cp = p.cls[k];
@@ -333,9 +353,11 @@ public:
 			C.block1 = cp.block1;
 			C.res_off = cp.res_off;
 			C.n_resume = cp.n_resume;
+			C.out_base = cp.out_base;
+			C.out_cap = cp.out_cap;
 			C.seg0 = cp.seg0;
 			C.seg1 = cp.seg1;
-			C.items = cp.items;
+			C.n_fresh = cp.n_fresh;
 		}
 		pack_segs(p.kseg, p, true, h_kseg);
 		pack_segs(p.gseg, p, false, h_gseg);
@@ -345,10 +367,19 @@ public:
 			                   hipMemcpyHostToDevice, A));
 		DCK(put_segs(h_kseg, d_kseg, p.kseg.size(), A));
 		DCK(put_segs(h_gseg, d_gseg, p.gseg.size(), A));
-		DCK(launch_srv_clear(d_ctr, A));
+		const unsigned long long budget =
+		    p.epoch_ms > 0.0 ? (unsigned long long)(p.epoch_ms * clock_khz) : 0ull;
+		DCK(launch_srv_clear(d_ctr, budget, A));
 		DCK(hipEventRecord(ev[0], A));
 		DCK(hipStreamWaitEvent(s[1], ev[0], 0));
 		DCK(hipStreamWaitEvent(s[2], ev[0], 0));
+
+		const bool run[3] = {p.n_cls > 0 && p.kerr_blocks > 0, p.n_fb > 0 && p.fb_blocks > 0,
+		                     p.grid_blocks > 0 && (p.g_resume > 0 || p.g_fresh > 0)};
+		SrvHeavy H;
+		H.total = (run[1] ? (unsigned)p.fb_blocks : 0u) + (run[2] ? (unsigned)p.grid_blocks : 0u);
+		H.epoch = ++epoch_no;
+		H.started = h_started;
 
 		SrvKerrArgs K;
 		K.reqs = d_reqs;
@@ -365,39 +396,54 @@ public:
 		SrvFbArgs F;
 		F.reqs = d_reqs;
 		F.in = fb[in];
+		F.in_off = p.fb_off;
 		F.n_in = p.n_fb;
 		F.out = fb[o];
 		F.ctr = d_ctr;
 		F.remaining = d_remaining;
 		F.done = d_done;
 		F.K = p.K_fb;
+		F.heavy = H;
 		SrvGridArgs Gd;
 		Gd.reqs = d_reqs;
 		Gd.seg = SrvSegs{d_gseg, d_gseg + SRV_MAX_SEGS, (const unsigned *)(d_gseg + 2 * SRV_MAX_SEGS)};
 		Gd.seg0 = 0;
 		Gd.seg1 = (int)p.gseg.size();
+		Gd.res_off = p.g_res_off;
 		Gd.n_resume = p.g_resume;
-		Gd.items = p.g_items;
+		Gd.n_fresh = p.g_fresh;
+		Gd.out_cap = p.g_out_cap;
 		Gd.in = tiles[in];
 		Gd.out = tiles[o];
 		Gd.ctr = d_ctr;
 		Gd.remaining = d_remaining;
 		Gd.done = d_done;
 		Gd.K = p.K_grid;
-		Gd.budget = p.grid_ms > 0.0 ? (unsigned long long)(p.grid_ms * clock_khz) : 0ull;
+		Gd.heavy = H;
 
-		const bool run[3] = {p.n_cls > 0 && p.kerr_blocks > 0, p.n_fb > 0, p.g_items > 0};
-		for (int k = 0; k < 3; ++k) {
+		/* the FP64 and grid launches first; the Kerr launch once all their
+		 * blocks are resident (or after a bound, past which the epoch's
+		 * stop time covers a launch that could not fit) */
+		for (int k = 1; k < 3; ++k) {
 			if (!run[k])
 				continue;
 			DCK(hipEventRecord(ev[1 + 2 * k], s[k]));
-			if (k == 0)
-				DCK(launch_srv_kerr(K, p.kerr_blocks, s[0]));
-			else if (k == 1)
-				DCK(launch_srv_fb64(F, s[1]));
+			if (k == 1)
+				DCK(launch_srv_fb64(F, p.fb_blocks, s[1]));
 			else
 				DCK(launch_srv_grid(Gd, p.grid_blocks, s[2]));
 			DCK(hipEventRecord(ev[2 + 2 * k], s[k]));
+		}
+		if (run[0]) {
+			if (H.total > 0) {
+				const auto t0 = std::chrono::steady_clock::now();
+				while (*h_started != H.epoch &&
+				       std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2))
+					;
+			}
+			DCK(hipEventRecord(ev[1], s[0]));
+			DCK(launch_srv_kerr(K, p.kerr_blocks, s[0]));
+			DCK(hipEventRecord(ev[2], s[0]));
 		}
 		for (int k = 0; k < 3; ++k)
 			DCK(hipStreamSynchronize(s[k]));
@@ -407,16 +453,20 @@ public:
 		if (h_ctr->n_done)
 			DCK(hipMemcpy(h_done, d_done, h_ctr->n_done * sizeof(int), hipMemcpyDeviceToHost));
 		for (int k = 0; k < SRV_MAX_CLASSES; ++k) {
-			out.claimed[k] = h_ctr->claimed[k];
+			out.res_claimed[k] = h_ctr->res_claimed[k];
+			out.fresh_claimed[k] = h_ctr->fresh_claimed[k];
 			out.saved[k] = h_ctr->saved[k];
 		}
-		out.fb_out = h_ctr->fb_out;
-		out.g_claimed = h_ctr->g_claimed;
+		out.fb_saved = h_ctr->fb_saved;
+		out.g_res = h_ctr->g_res;
+		out.g_fresh = h_ctr->g_fresh;
 		out.g_saved = h_ctr->g_saved;
+		out.overflow = h_ctr->overflow;
 		out.done.assign(h_done, h_done + h_ctr->n_done);
 		for (int k = 0; k < 3; ++k) {
 			out.steps[k] = h_ctr->steps[k];
 			out.ms[k] = -1.0;
+			out.late[k] = h_ctr->late[k];
 		}
 		/* kernel times in scheduler order: Kerr, FP64, grid */
 		for (int k = 0; k < 3; ++k) {

@@ -77,7 +77,59 @@ __device__ __forceinline__ void seg_item(const SrvSegs &S, int s0, int s1, unsig
 	first = S.first[lo] + (int)(f - S.start[lo]);
 }
 
+/* The epoch's stop time, as the clear kernel set it before this launch;
+ * a wave that starts at or after it is counted in late[kind] (the host
+ * learns how much of a launch could not run beside the others). */
+__device__ __forceinline__ unsigned long long epoch_end(SrvCtr *ctr, int kind)
+{
+	const unsigned long long e = *(const volatile unsigned long long *)&ctr->t_end;
+	const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)e);
+	const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(e >> 32));
+	const unsigned long long end = ((unsigned long long)hi << 32) | lo;
+	if (__lane_id() == 0 && (unsigned long long)wall_clock64() >= end)
+		atomicAdd(&ctr->late[kind], 1u);
+	return end;
+}
+
 /* ---------------- analytic Kerr, FP32 form ---------------- */
+
+/* Wave-wide claim of one item per asking lane from a cursor: the lane's
+ * cursor value (UINT_MAX for a lane not asking). */
+__device__ __forceinline__ unsigned claim_items(unsigned *cursor, bool want)
+{
+	const unsigned long long m = __ballot(want);
+	if (!m)
+		return UINT_MAX;
+	const unsigned lane = __lane_id();
+	const int leader = __ffsll((long long)m) - 1;
+	unsigned base = 0;
+	if ((int)lane == leader)
+		base = atomicAdd(cursor, (unsigned)__popcll(m));
+	base = __shfl(base, leader);
+	return want ? base + (unsigned)__popcll(m & ((1ull << lane) - 1ull)) : UINT_MAX;
+}
+
+/* Slot of the next entry written into region [base, base + cap) of a list,
+ * from the back; -1 (counted as an overflow) past its start. */
+__device__ __forceinline__ int region_slot(unsigned *counter, int base, int cap, SrvCtr *ctr)
+{
+	const unsigned k = atomicAdd(counter, 1u);
+	if (k >= (unsigned)cap) {
+		atomicAdd(&ctr->overflow, 1u);
+		return -1;
+	}
+	return base + cap - 1 - (int)k;
+}
+
+/* Kerr resume entry i of list `in` into slot o of list `out`, unchanged. */
+__device__ __forceinline__ void copy_resume(const SrvResume &in, int i, const SrvResume &out, int o)
+{
+	out.meta[o] = in.meta[i];
+	out.steps[o] = in.steps[i];
+#pragma unroll
+	for (int r = 0; r < 2 * SRV_NK; ++r)
+		out.st[(size_t)r * out.cap + o] = in.st[(size_t)r * in.cap + i];
+}
 
 template <int NF>
 __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
@@ -85,14 +137,12 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
                                            const SrvClass &C, int c)
 {
 	constexpr int ND = nd32<NF>();
-	const unsigned lane = __lane_id();
 	const RtConst<NF> rc = make_rtconst<NF>(C);
 	const StepRule sr = C.sr;
 	const float dts = (float)sr.dts;
 	const double dt_over_n = sr.dt / (double)sr.n;
 	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
-	const unsigned items = C.items;
-	const int n_resume = C.n_resume;
+	const unsigned n_resume = (unsigned)C.n_resume, n_fresh = C.n_fresh;
 	LaneKerr P;
 	P.r_esc = (float)C.r_esc;
 	P.escape = C.escape;
@@ -102,63 +152,68 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 	int tab = -1, ray = 0, d = 0, h = 0;
 	int64_t st = 0, st0 = 0;
 	bool exhausted = false;
+	bool res_left = n_resume > 0;   /* uniform: resumed rays may be unclaimed */
+	const unsigned long long end = epoch_end(A.ctr, 0);
 	for (int it = 0;; ++it) {
-		const bool stop = it >= A.K;
-		const bool need = tab < 0 && !exhausted && !stop;
-		const unsigned long long m = __ballot(need);
-		if (m) {
-			const int leader = __ffsll((unsigned long long)m) - 1;
-			unsigned base = 0;
-			if ((int)lane == leader)
-				base = atomicAdd(&A.ctr->claimed[c], (unsigned)__popcll(m));
-			base = __shfl(base, leader);
-			if (need) {
-				const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-				if (idx >= items) {
-					exhausted = true;
-				} else if ((int)idx < n_resume) {
-					const int e = C.res_off + (int)idx;
-					const int4 mt = A.in.meta[e];
-					tab = mt.x;
-					ray = mt.y;
-					d = mt.z;
-					h = mt.w;
-					st = A.in.steps[e];
+		const bool stop = it >= A.K || (unsigned long long)wall_clock64() >= end;
+		bool need = tab < 0 && !exhausted && !stop;
+		const bool took = need;   /* a lane asking gets a ray or is exhausted */
+		/* refill: resumed rays first, then fresh ones */
+		if (res_left) {
+			const unsigned idx = claim_items(&A.ctr->res_claimed[c], need);
+			if (__ballot(need && idx >= n_resume))
+				res_left = false;
+			if (need && idx < n_resume) {
+				const int e = C.res_off + (int)idx;
+				const int4 mt = A.in.meta[e];
+				tab = mt.x;
+				ray = mt.y;
+				d = mt.z;
+				h = mt.w;
+				st = A.in.steps[e];
 #pragma unroll
-					for (int i = 0; i < ND; ++i) {
-						hi[i] = A.in.st[(size_t)i * A.in.cap + e];
-						lo[i] = A.in.st[(size_t)(SRV_NK + i) * A.in.cap + e];
-					}
-				} else {
-					seg_item(A.seg, C.seg0, C.seg1, idx - (unsigned)n_resume, tab, ray);
-					double x[8 + 2 * NF];
-					icond_ray<NF, RayParams>(reqs[tab].p, (size_t)ray, x);
-					split_state<NF, SrvClass>(C, x, hi, lo);
-					d = 0;
-					h = 0;
-					st = 0;
+				for (int i = 0; i < ND; ++i) {
+					hi[i] = A.in.st[(size_t)i * A.in.cap + e];
+					lo[i] = A.in.st[(size_t)(SRV_NK + i) * A.in.cap + e];
 				}
-				if (tab >= 0) {
-					const RayParams &p = reqs[tab].p;
-					P.a = (float)p.a;
-					P.r_h = (float)p.r_h;
-					P.eps_stop = (float)p.eps_stop;
-					P.r_isco = (float)p.r_isco;
-					st0 = st;
-				}
+				need = false;
+			}
+		}
+		const unsigned fidx = claim_items(&A.ctr->fresh_claimed[c], need);
+		if (need) {
+			if (fidx >= n_fresh) {
+				exhausted = true;
+			} else {
+				seg_item(A.seg, C.seg0, C.seg1, fidx, tab, ray);
+				double x[8 + 2 * NF];
+				icond_ray<NF, RayParams>(reqs[tab].p, (size_t)ray, x);
+				split_state<NF, SrvClass>(C, x, hi, lo);
+				d = 0;
+				h = 0;
+				st = 0;
 			}
 		}
 		if (tab < 0)
 			break;
+		if (took) {
+			const RayParams &p = reqs[tab].p;
+			P.a = (float)p.a;
+			P.r_h = (float)p.r_h;
+			P.eps_stop = (float)p.eps_stop;
+			P.r_isco = (float)p.r_isco;
+			st0 = st;
+		}
 		if (stop) {
-			/* iterations used up: the ray goes on next epoch */
-			const int e = C.block0 * SRV_BLOCK + (int)atomicAdd(&A.ctr->saved[c], 1u);
-			A.out.meta[e] = make_int4(tab, ray, d, h);
-			A.out.steps[e] = st;
+			/* the ray goes on next epoch */
+			const int e = region_slot(&A.ctr->saved[c], C.out_base, C.out_cap, A.ctr);
+			if (e >= 0) {
+				A.out.meta[e] = make_int4(tab, ray, d, h);
+				A.out.steps[e] = st;
 #pragma unroll
-			for (int i = 0; i < ND; ++i) {
-				A.out.st[(size_t)i * A.out.cap + e] = hi[i];
-				A.out.st[(size_t)(SRV_NK + i) * A.out.cap + e] = lo[i];
+				for (int i = 0; i < ND; ++i) {
+					A.out.st[(size_t)i * A.out.cap + e] = hi[i];
+					A.out.st[(size_t)(SRV_NK + i) * A.out.cap + e] = lo[i];
+				}
 			}
 			atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
 			break;
@@ -217,8 +272,8 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			x[8 + SRV_NF] = NF > 0 ? (double)hi[ND - 1] + (double)lo[ND - 1] : 0.0;
 			/* the host plans the list's room; a count past it is reported
 			 * as an error and nothing is written out of bounds */
-			const unsigned e = atomicAdd(&A.ctr->fb_out, 1u);
-			if (e < (unsigned)A.fb.cap) {
+			const int e = region_slot(&A.ctr->fb_saved, 0, A.fb.cap, A.ctr);
+			if (e >= 0) {
 				A.fb.meta[e] = make_int4(tab, ray, d, state == 3 ? -1 : h);
 				A.fb.steps[e] = st;
 #pragma unroll
@@ -228,7 +283,19 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 		}
 		tab = -1;
 	}
+	/* carry the resumed rays nobody claimed over to the next epoch */
+	while (res_left) {
+		const unsigned idx = claim_items(&A.ctr->res_claimed[c], true);
+		if (__ballot(idx >= n_resume))
+			res_left = false;
+		if (idx < n_resume) {
+			const int o = region_slot(&A.ctr->saved[c], C.out_base, C.out_cap, A.ctr);
+			if (o >= 0)
+				copy_resume(A.in, C.res_off + (int)idx, A.out, o);
+		}
+	}
 }
+
 
 /* The request and class tables come as restrict parameters besides A: the
  * kernels write only through other pointers, so their loads stay scalar. */
@@ -254,75 +321,127 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A,
 
 /* ---------------- FP64 re-integration ---------------- */
 
+/* The last FP64 or grid block to start tells the host, which then launches
+ * the Kerr kernel into the room left (host memory, written once). */
+__device__ __forceinline__ void heavy_started(const SrvHeavy &H, SrvCtr *ctr)
+{
+	if (threadIdx.x == 0 && atomicAdd(&ctr->heavy_started, 1u) + 1u == H.total) {
+		*H.started = H.epoch;
+		__threadfence_system();
+	}
+}
+
+/* FP64 entry i of list `in` into slot o of list `out`, unchanged. */
+__device__ __forceinline__ void copy_fb(const SrvFb &in, int i, const SrvFb &out, int o)
+{
+	out.meta[o] = in.meta[i];
+	out.steps[o] = in.steps[i];
+#pragma unroll
+	for (int c = 0; c < SRV_NC; ++c)
+		out.x[(size_t)c * out.cap + o] = in.x[(size_t)c * in.cap + i];
+}
+
+/* FP64 re-integration: a persistent pool whose lanes refill from the list
+ * as rays finish; at the stop time survivors, then the entries nobody
+ * claimed, go to the out-list. */
 __global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A,
                                                         const SrvRequest *__restrict__ reqs)
 {
-	const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-	if (e >= A.n_in)
-		return;
-	const int4 mt = A.in.meta[e];
-	const int tab = mt.x, ray = mt.y;
-	int d = mt.z, h = mt.w;
-	const SrvRequest &R = reqs[tab];
-	const StepRule sr = R.sr;
-	const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
+	heavy_started(A.heavy, A.ctr);
+	const unsigned long long end = epoch_end(A.ctr, 1);
+	const unsigned n_in = (unsigned)A.n_in;
+	int tab = -1, ray = 0, d = 0, h = 0;
 	double x[SRV_NC];
-	int64_t st = A.in.steps[e];
-	if (h < 0) {
-		/* non-finite FP32 state: the whole ray again in FP64 */
-		icond_ray<SRV_NF, RayParams>(R.p, (size_t)ray, x);
-		d = 0;
-		h = 0;
-		st = 0;
-	} else {
+	int64_t st = 0, st0 = 0;
+	bool exhausted = false;
+	for (int it = 0;; ++it) {
+		const bool stop = it >= A.K || (unsigned long long)wall_clock64() >= end;
+		const bool need = tab < 0 && !exhausted && !stop;
+		const unsigned idx = claim_items(&A.ctr->fb_claimed, need);
+		if (need) {
+			if (idx >= n_in) {
+				exhausted = true;
+			} else {
+				const int e = A.in_off + (int)idx;
+				const int4 mt = A.in.meta[e];
+				tab = mt.x;
+				ray = mt.y;
+				d = mt.z;
+				h = mt.w;
+				st = A.in.steps[e];
+				if (h < 0) {
+					/* non-finite FP32 state: the whole ray again in FP64 */
+					icond_ray<SRV_NF, RayParams>(reqs[tab].p, (size_t)ray, x);
+					d = 0;
+					h = 0;
+					st = 0;
+				} else {
 #pragma unroll
-		for (int c = 0; c < SRV_NC; ++c)
-			x[c] = A.in.x[(size_t)c * A.in.cap + e];
-	}
-	/* x[8 + SRV_NF] holds the channels' shared depth T (0 from the camera) */
-	const int64_t st0 = st;
-	const double n_over_dt = (double)sr.n / sr.dt;
-	bool fin = false;
-	for (int k = 0; k < A.K; ++k) {
-		V4<double> q = {x[0], x[1], x[2], x[3]};
-		V4<double> u = {x[4], x[5], x[6], x[7]};
-		const double xd = getdt(R.p, q, u, sr.dts) * n_over_dt;
-		int dh = (xd == xd) ? (int)xd : 0;
-		if (!dh) {
-			fin = true;
-			break;
-		}
-		if (dh > sr.n - h)
-			dh = sr.n - h;
-		rk4_fb<SRV_NF, RayParams>(R.p, rc, x, (double)dh * sr.dt / (double)sr.n);
-		++st;
-		h += dh;
-		if (h >= sr.n) {
-			h = 0;
-			if (++d >= sr.n_dump) {
-				fin = true;
-				break;
+					for (int c = 0; c < SRV_NC; ++c)
+						x[c] = A.in.x[(size_t)c * A.in.cap + e];
+				}
+				st0 = st;
 			}
 		}
-	}
-	atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
-	if (fin) {
-		const double T = x[8 + SRV_NF];
+		if (tab < 0)
+			break;
+		const SrvRequest &R = reqs[tab];
+		if (stop) {
+			const int o = region_slot(&A.ctr->fb_saved, 0, A.out.cap, A.ctr);
+			if (o >= 0) {
+				A.out.meta[o] = make_int4(tab, ray, d, h);
+				A.out.steps[o] = st;
 #pragma unroll
-		for (int i = 0; i < SRV_NF; ++i)
-			x[8 + SRV_NF + i] = rc.d[i] * T;
-		V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
-		finish_ray(R, tab, ray, x, SRV_NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr, A.done);
-		return;
-	}
-	const unsigned o = atomicAdd(&A.ctr->fb_out, 1u);
-	if (o >= (unsigned)A.out.cap)
-		return;   /* reported by the host as an overflow */
-	A.out.meta[o] = make_int4(tab, ray, d, h);
-	A.out.steps[o] = st;
+				for (int c = 0; c < SRV_NC; ++c)
+					A.out.x[(size_t)c * A.out.cap + o] = x[c];
+			}
+			atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
+			break;
+		}
+		/* x[8 + SRV_NF] holds the channels' shared depth T (0 from the camera) */
+		const StepRule sr = R.sr;
+		V4<double> q = {x[0], x[1], x[2], x[3]};
+		V4<double> u = {x[4], x[5], x[6], x[7]};
+		const double xd = getdt(R.p, q, u, sr.dts) * ((double)sr.n / sr.dt);
+		int dh = (xd == xd) ? (int)xd : 0;
+		bool fin = !dh;
+		if (!fin) {
+			if (dh > sr.n - h)
+				dh = sr.n - h;
+			const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
+			rk4_fb<SRV_NF, RayParams>(R.p, rc, x, (double)dh * sr.dt / (double)sr.n);
+			++st;
+			h += dh;
+			if (h >= sr.n) {
+				h = 0;
+				fin = ++d >= sr.n_dump;
+			}
+		}
+		if (fin) {
+			atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
+			const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
+			const double T = x[8 + SRV_NF];
 #pragma unroll
-	for (int c = 0; c < SRV_NC; ++c)
-		A.out.x[(size_t)c * A.out.cap + o] = x[c];
+			for (int i = 0; i < SRV_NF; ++i)
+				x[8 + SRV_NF + i] = rc.d[i] * T;
+			V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
+			finish_ray(R, tab, ray, x, SRV_NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr,
+			           A.done);
+			tab = -1;
+		}
+	}
+	/* carry the entries nobody claimed over to the next epoch */
+	bool left = !exhausted;
+	while (__ballot(left)) {
+		const unsigned idx = claim_items(&A.ctr->fb_claimed, true);
+		if (__ballot(idx >= n_in))
+			left = false;
+		if (idx < n_in) {
+			const int o = region_slot(&A.ctr->fb_saved, 0, A.out.cap, A.ctr);
+			if (o >= 0)
+				copy_fb(A.in, A.in_off + (int)idx, A.out, o);
+		}
+	}
 }
 
 /* ---------------- gridded spacetimes, FP32 form ---------------- */
@@ -476,11 +595,13 @@ __device__ __forceinline__ bool grid_tile(const SrvGridArgs &A, const SrvRequest
 	const bool save = __ballot(live) != 0;
 	if (save) {
 		/* iterations or time used up: the tile goes on next epoch */
-		unsigned r = 0;
+		int r = 0;
 		if (lane == 0)
-			r = atomicAdd(&A.ctr->g_saved, 1u);
-		r = __shfl(r, 0);
-		const int e = (int)r * 64 + (int)lane;
+			r = region_slot(&A.ctr->g_saved, 0, A.out_cap, A.ctr);
+		r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
+		if (r < 0)
+			return true;   /* counted as an overflow */
+		const int e = r * 64 + (int)lane;
 		if (lane == 0) {
 			A.out.hdr[r] = make_int2(tab, tile);
 			A.out.brick[r] = make_int4(hdr->o[0], hdr->o[1], hdr->o[2], hdr->cool);
@@ -498,80 +619,109 @@ __device__ __forceinline__ bool grid_tile(const SrvGridArgs &A, const SrvRequest
 	return save;
 }
 
-/* The launch's stop time: the first wave to start sets it to its start plus
- * the budget, so that every wave stops at about the same moment and none
- * idles while another finishes a fixed count of slower iterations. */
-__device__ __forceinline__ unsigned long long grid_stop_time(const SrvGridArgs &A)
+/* Grid record i of list `in` into record o of list `out`, unchanged (the
+ * whole wave copies its 64 lanes). */
+__device__ __forceinline__ void copy_tile(const SrvTiles &in, int i, const SrvTiles &out, int o)
 {
-	if (A.budget == 0)
-		return ~0ull;
-	unsigned long long end = 0;
-	if (__lane_id() == 0) {
-		const unsigned long long want = (unsigned long long)wall_clock64() + A.budget;
-		const unsigned long long old = atomicCAS(&A.ctr->g_end, 0ull, want);
-		end = old ? old : want;
+	const unsigned lane = __lane_id();
+	if (lane == 0) {
+		out.hdr[o] = in.hdr[i];
+		out.brick[o] = in.brick[i];
 	}
-	const unsigned lo = __builtin_amdgcn_readfirstlane(__shfl((unsigned)end, 0));
-	const unsigned hi = __builtin_amdgcn_readfirstlane(__shfl((unsigned)(end >> 32), 0));
-	return ((unsigned long long)hi << 32) | lo;
+	const int a = i * 64 + (int)lane, b = o * 64 + (int)lane;
+	out.meta[b] = in.meta[a];
+	out.steps[b] = in.steps[a];
+#pragma unroll
+	for (int r = 0; r < 2 * SRV_NG; ++r)
+		out.st[(size_t)r * (64 * (size_t)out.cap) + b] = in.st[(size_t)r * (64 * (size_t)in.cap) + a];
 }
 
 /* Every gridded request's tiles, built for SRV_NF channels and run with the
  * request's count (uniform per wave; the others cost no transcendentals):
- * two waves per SIMD, as the LDS bricks allow.  A wave claims tiles until
- * the queue is empty or its current tile is saved; it claims at least once,
- * so the resumed records (the first items, no more than the waves) are all
- * taken. */
+ * two waves per SIMD, as the LDS bricks allow.  A wave claims tiles
+ * (resumed records first, then fresh tiles) until none are left, its
+ * current tile is saved or the stop time passes; then it carries the
+ * resumed records nobody claimed over. */
 __global__ __launch_bounds__(64 * SRV_GWAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void k_srv_grid(SrvGridArgs A, const SrvRequest *__restrict__ reqs)
 {
 	__shared__ float4 s_brick[SRV_GWAVES][GP_PTS * 10];
 	__shared__ BrickHdr s_hdr[SRV_GWAVES];
+	heavy_started(A.heavy, A.ctr);
 	const unsigned lane = __lane_id();
 	const int wv = (int)(threadIdx.x >> 6);
 	float4 *brick = s_brick[wv];
 	BrickHdr *hdr = &s_hdr[wv];
 	int it = 0;
-	const unsigned long long end = grid_stop_time(A);
-	for (;;) {
-		unsigned item = 0;
-		if (lane == 0)
-			item = atomicAdd(&A.ctr->g_claimed, 1u);
-		item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
-		if (item >= A.items)
-			break;
-		int tab, tile, rec = -1;
-		const bool resumed = (int)item < A.n_resume;
-		if (resumed) {
-			rec = (int)item;
-			const int2 hd = A.in.hdr[rec];
-			tab = hd.x;
-			tile = hd.y;
-		} else {
-			seg_item(A.seg, A.seg0, A.seg1, item - (unsigned)A.n_resume, tab, tile);
+	const unsigned n_resume = (unsigned)A.n_resume;
+	bool res_left = n_resume > 0;
+	const unsigned long long end = epoch_end(A.ctr, 2);
+	while (it < A.K && (unsigned long long)wall_clock64() < end) {
+		int tab = -1, tile = 0, rec = -1;
+		unsigned r = UINT_MAX;
+		if (res_left) {
+			if (lane == 0)
+				r = atomicAdd(&A.ctr->g_res, 1u);
+			r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
+			if (r < n_resume) {
+				rec = A.res_off + (int)r;
+				const int2 hd = A.in.hdr[rec];
+				tab = hd.x;
+				tile = hd.y;
+			} else {
+				res_left = false;
+			}
+		}
+		if (rec < 0) {
+			unsigned f = 0;
+			if (lane == 0)
+				f = atomicAdd(&A.ctr->g_fresh, 1u);
+			f = __builtin_amdgcn_readfirstlane(__shfl(f, 0));
+			if (f >= A.n_fresh)
+				break;
+			seg_item(A.seg, A.seg0, A.seg1, f, tab, tile);
 		}
 		tab = __builtin_amdgcn_readfirstlane(tab);
 		tile = __builtin_amdgcn_readfirstlane(tile);
 		const int nf = __builtin_amdgcn_readfirstlane(reqs[tab].p.n_freq);
-		if (grid_tile<SRV_NF>(A, reqs, tab, tile, resumed, rec, brick, hdr, it, end, nf) ||
-		    it >= A.K || (unsigned long long)wall_clock64() >= end)
+		if (grid_tile<SRV_NF>(A, reqs, tab, tile, rec >= 0, rec, brick, hdr, it, end, nf))
 			break;
+	}
+	/* carry the resumed records nobody claimed over to the next epoch */
+	while (res_left) {
+		unsigned r = 0;
+		if (lane == 0)
+			r = atomicAdd(&A.ctr->g_res, 1u);
+		r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
+		if (r >= n_resume)
+			break;
+		int o = 0;
+		if (lane == 0)
+			o = region_slot(&A.ctr->g_saved, 0, A.out_cap, A.ctr);
+		o = __builtin_amdgcn_readfirstlane(__shfl(o, 0));
+		if (o >= 0)
+			copy_tile(A.in, A.res_off + (int)r, A.out, o);
 	}
 }
 
-__global__ void k_srv_clear(SrvCtr *ctr)
+/* Clears the epoch's counters and starts its clock: the Kerr and grid
+ * launches stop at t_end, so that every wave stops at about the same
+ * moment (a count of iterations would leave the fast waves idle while the
+ * slowest finishes, and would run blocks that start late past the others). */
+__global__ void k_srv_clear(SrvCtr *ctr, unsigned long long budget)
 {
-	if (threadIdx.x == 0)
-		*ctr = SrvCtr{};
+	if (threadIdx.x == 0) {
+		SrvCtr c{};
+		c.t_end = budget ? (unsigned long long)wall_clock64() + budget : ~0ull;
+		*ctr = c;
+	}
 }
 
 /* ---------------- launchers ---------------- */
 
-static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-hipError_t launch_srv_clear(SrvCtr *ctr, hipStream_t st)
+hipError_t launch_srv_clear(SrvCtr *ctr, unsigned long long budget, hipStream_t st)
 {
-	hipLaunchKernelGGL(k_srv_clear, dim3(1), dim3(64), 0, st, ctr);
+	hipLaunchKernelGGL(k_srv_clear, dim3(1), dim3(64), 0, st, ctr, budget);
 	return hipGetLastError();
 }
 
@@ -583,18 +733,17 @@ hipError_t launch_srv_kerr(const SrvKerrArgs &A, int n_blocks, hipStream_t st)
 	return hipGetLastError();
 }
 
-hipError_t launch_srv_fb64(const SrvFbArgs &A, hipStream_t st)
+hipError_t launch_srv_fb64(const SrvFbArgs &A, int n_blocks, hipStream_t st)
 {
-	if (A.n_in <= 0)
+	if (n_blocks <= 0 || A.n_in <= 0)
 		return hipSuccess;
-	hipLaunchKernelGGL(k_srv_fb64, dim3(blocks_for((size_t)A.n_in, SRV_BLOCK)), dim3(SRV_BLOCK),
-	                   0, st, A, A.reqs);
+	hipLaunchKernelGGL(k_srv_fb64, dim3(n_blocks), dim3(SRV_BLOCK), 0, st, A, A.reqs);
 	return hipGetLastError();
 }
 
 hipError_t launch_srv_grid(const SrvGridArgs &A, int n_blocks, hipStream_t st)
 {
-	if (n_blocks <= 0 || A.items == 0)
+	if (n_blocks <= 0 || (A.n_resume <= 0 && A.n_fresh == 0))
 		return hipSuccess;
 	hipLaunchKernelGGL(k_srv_grid, dim3(n_blocks), dim3(64 * SRV_GWAVES), 0, st, A, A.reqs);
 	return hipGetLastError();
@@ -608,6 +757,25 @@ hipError_t srv_kerr_blocks_per_cu(int *blocks)
 hipError_t srv_grid_blocks_per_cu(int *blocks)
 {
 	return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_srv_grid, 64 * SRV_GWAVES, 0);
+}
+
+/* Kerr blocks a CU holds beside one FP64 or grid block: every block has a
+ * wave on each SIMD, whose 512 registers per lane the waves share
+ * (allocated in granules of 8). */
+hipError_t srv_kerr_beside_heavy(int *blocks)
+{
+	hipFuncAttributes k, g, f;
+	hipError_t e = hipFuncGetAttributes(&k, reinterpret_cast<const void *>(&k_srv_kerr));
+	if (e == hipSuccess)
+		e = hipFuncGetAttributes(&g, reinterpret_cast<const void *>(&k_srv_grid));
+	if (e == hipSuccess)
+		e = hipFuncGetAttributes(&f, reinterpret_cast<const void *>(&k_srv_fb64));
+	if (e != hipSuccess)
+		return e;
+	auto up8 = [](int r) { return (r + 7) / 8 * 8; };
+	const int rk = up8(k.numRegs), rh = up8(g.numRegs > f.numRegs ? g.numRegs : f.numRegs);
+	*blocks = (rk > 0 && rh > 0 && rh < 512) ? (512 - rh) / rk : 0;
+	return hipSuccess;
 }
 
 } // namespace gray

@@ -2,8 +2,8 @@
  * serve.h -- device structures of the serving pool.
  *
  * Requests live in a device table.  Each epoch runs three persistent
- * kernels side by side, each with its own work queue and a bounded number
- * of iterations per wave:
+ * kernels side by side until the epoch's stop time (wall clock), each with
+ * its own work queue:
  *   analytic Kerr, FP32 form  one lane per ray; lanes refill from their
  *                             class's queue (rays resumed from the previous
  *                             epoch first, then fresh rays of the class's
@@ -12,9 +12,13 @@
  *                             with its LDS brick; waves claim tiles (resumed
  *                             tiles first, then fresh ones);
  *   FP64 re-integration       one lane per ray handed over by the Kerr
- *                             kernel (collapsed step or non-finite state).
- * A ray or tile still running when its wave's iterations are used up is
- * saved to the next epoch's resume list, state bits intact.
+ *                             kernel (collapsed step or non-finite state);
+ *                             lanes refill from the list.
+ * At the stop time a lane or wave saves what it holds, state bits intact,
+ * and then carries the resumed entries nobody claimed over unchanged, so a
+ * launch may have fewer lanes than it has entries to resume.  Each list is
+ * written from the back of its region: the next epoch claims in index
+ * order, so the entries written last (those carried over) come first.
  *
  * Kerr requests are grouped in classes that share every parameter except
  * the spin and the camera (channels, step schedule, fluid, escape rule), so
@@ -53,10 +57,10 @@ struct SrvClass {
 	double r_esc;
 	int n_freq, fluid, escape, pad;
 	int block0, block1;   /* blocks [block0, block1) serve this class        */
-	int res_off, n_resume;/* resume entries [res_off, res_off + n_resume)    */
-	int seg0, seg1;       /* fresh segments [seg0, seg1); seg_start[seg1] is
-	                         the class's fresh ray count                     */
-	unsigned items;       /* n_resume + fresh rays                           */
+	int res_off, n_resume;/* resumed rays: in-list [res_off, res_off + n)    */
+	int out_base, out_cap;/* its region of the out-list                      */
+	int seg0, seg1;       /* fresh segments [seg0, seg1)                     */
+	unsigned n_fresh;     /* fresh rays in them                              */
 };
 
 /* Rays saved at the end of a Kerr epoch (SoA, capacity cap). */
@@ -90,14 +94,20 @@ struct SrvTiles {
 
 /* Counters of one epoch (cleared by a kernel before it). */
 struct SrvCtr {
-	unsigned claimed[SRV_MAX_CLASSES];  /* Kerr queue claims per class     */
-	unsigned saved[SRV_MAX_CLASSES];    /* Kerr rays saved per class        */
-	unsigned fb_out;                    /* entries in the FP64 out list     */
-	unsigned g_claimed, g_saved;        /* grid tile claims, records saved  */
+	unsigned res_claimed[SRV_MAX_CLASSES];   /* resumed-ray cursor per class  */
+	unsigned fresh_claimed[SRV_MAX_CLASSES]; /* fresh-ray cursor per class    */
+	unsigned saved[SRV_MAX_CLASSES];         /* rays written to the out region */
+	unsigned fb_claimed, fb_saved;      /* FP64 list: cursor; out-list entries
+	                                       (survivors, carried, hand-overs) */
+	unsigned g_res, g_fresh, g_saved;   /* grid: cursors; records written   */
 	unsigned n_done;                    /* requests finished this epoch     */
+	unsigned overflow;                  /* writes refused past a region     */
+	unsigned heavy_started;             /* FP64 and grid blocks started     */
+	unsigned late[3];                   /* waves that started at or after
+	                                       t_end: Kerr, FP64, grid          */
 	unsigned long long steps[3];        /* FP32 Kerr, FP64 Kerr, grid       */
-	unsigned long long g_end;           /* grid launch's stop time (wall
-	                                       clock ticks; 0 until set)        */
+	unsigned long long t_end;           /* the epoch's stop time (wall clock
+	                                       ticks; all ones: none)           */
 };
 
 /* Fresh work: segment j covers queue items [start[j], start[j+1]) of its
@@ -114,37 +124,47 @@ struct SrvKerrArgs {
 	int n_cls;
 	SrvSegs seg;
 	SrvResume in, out;
-	SrvFb fb;             /* hand-overs are appended here                   */
+	SrvFb fb;             /* the FP64 out-list: hand-overs are written here */
 	SrvCtr *ctr;
 	unsigned *remaining;  /* per table index: rays not yet finished        */
 	int *done;            /* table indices of finished requests             */
-	int K;                /* iterations per wave                            */
+	int K;                /* bound on iterations per wave                   */
+};
+
+/* The FP64 and grid launches go first; the last of their blocks to start
+ * writes `epoch` to *started (host memory), after which the host launches
+ * the Kerr kernel into the room they left. */
+struct SrvHeavy {
+	unsigned total;       /* FP64 + grid blocks of the epoch               */
+	unsigned epoch;
+	volatile unsigned *started;
 };
 
 struct SrvGridArgs {
 	const SrvRequest *reqs;
 	SrvSegs seg;
 	int seg0, seg1;       /* fresh tile segments                           */
-	int n_resume;         /* resumed tile records (first items)            */
-	unsigned items;       /* n_resume + fresh tiles                        */
+	int res_off, n_resume;/* resumed records [res_off, res_off + n_resume)  */
+	unsigned n_fresh;     /* fresh tiles                                   */
+	int out_cap;          /* records of the out region [0, out_cap)         */
 	SrvTiles in, out;
 	SrvCtr *ctr;
 	unsigned *remaining;
 	int *done;
 	int K;
-	unsigned long long budget;   /* wall clock ticks from the first wave's start
-	                                to the stop (0: K iterations only)    */
+	SrvHeavy heavy;
 };
 
 struct SrvFbArgs {
 	const SrvRequest *reqs;
 	SrvFb in;
-	int n_in;
-	SrvFb out;            /* survivors are appended here                    */
+	int in_off, n_in;     /* entries [in_off, in_off + n_in) of the in-list */
+	SrvFb out;            /* survivors and carried entries are written here */
 	SrvCtr *ctr;
 	unsigned *remaining;
 	int *done;
 	int K;
+	SrvHeavy heavy;
 };
 
 } // namespace gray

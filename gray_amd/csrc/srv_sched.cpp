@@ -23,7 +23,7 @@ constexpr double PRIOR_UNITS[2] = {4096.0, 64.0};   /* weight of the priors */
 constexpr size_t KERR_CHUNK = 256;   /* rays per request per queue round   */
 constexpr size_t GRID_CHUNK = 4;     /* tiles per request per queue round  */
 constexpr size_t GRID_FRESH = 4;     /* fresh tiles offered per grid wave  */
-constexpr int GRID_K_BOUND = 1 << 20;
+constexpr int K_BOUND = 1 << 20;     /* iterations bound of timed launches */
 
 void set_err(char *err, size_t len, const char *msg)
 {
@@ -82,7 +82,8 @@ void Sched::Arena::release(size_t off, size_t bytes)
 }
 
 Sched::Sched(const Config &cfg, const Caps &caps, Device *dev)
-    : cfg_(cfg), caps_(caps), dev_(dev), g_resume_(0), n_fb_(0), flip_(0), next_id_(1),
+    : cfg_(cfg), caps_(caps), dev_(dev), g_res_off_(0), g_resume_(0), fb_off_(0), n_fb_(0),
+      flip_(0), next_id_(1),
       epochs_(0)
 {
 	arena_.init(caps.arena);
@@ -212,19 +213,23 @@ void Sched::plan_locked(Plan &p)
 	p.n_cls = 0;
 	p.kseg.clear();
 	p.gseg.clear();
-	p.kerr_blocks = p.grid_blocks = 0;
-	p.g_resume = 0;
-	p.g_items = 0;
-	p.flip = flip_;
-	p.K_kerr = K_[0];
-	p.K_fb = K_[1];
-	/* the grid launch stops on time rather than after a count of
-	 * iterations: its waves' iterations differ in cost by the cells they
-	 * read, and a count would leave the fast ones idle while the slowest
-	 * finishes (K_grid stays as a bound) */
-	p.grid_ms = cfg_.epoch_ms;
-	p.K_grid = cfg_.epoch_ms > 0 ? GRID_K_BOUND : K_[2];
+	p.kerr_blocks = p.grid_blocks = p.fb_blocks = 0;
+	p.g_res_off = g_res_off_;
+	p.g_resume = g_resume_;
+	p.g_out_cap = 0;
+	p.g_fresh = 0;
+	p.fb_off = fb_off_;
 	p.n_fb = n_fb_;
+	p.flip = flip_;
+	/* the launches stop on time rather than after a count of iterations
+	 * (their K stay as bounds): a gridded wave's iterations differ in cost
+	 * by the cells it reads, and a count would leave the fast waves idle
+	 * while the slowest finishes */
+	p.epoch_ms = cfg_.epoch_ms;
+	const bool timed = cfg_.epoch_ms > 0;
+	p.K_kerr = timed ? K_BOUND : K_[0];
+	p.K_fb = timed ? K_BOUND : K_[1];
+	p.K_grid = timed ? K_BOUND : K_[2];
 
 	/* work left per class and for the grid kind */
 	double wk[SRV_MAX_CLASSES] = {0};
@@ -256,31 +261,54 @@ void Sched::plan_locked(Plan &p)
 	}
 	const double tiles_left = wg + g_resume_;
 
-	/* the GPU shared between the two launches by the time their work left
-	 * would take on all of it: units x steps per unit x block-ms per step /
-	 * blocks the GPU holds */
-	int kerr_total = caps_.kerr_blocks, grid_total = caps_.grid_blocks;
-	if (wk_all > 0 && tiles_left > 0) {
-		const double tk = wk_all * spu(0) * bms_[0] / caps_.kerr_blocks;
-		const double tg = tiles_left * spu(1) * bms_[1] / caps_.grid_blocks;
-		const double fg = tg / (tg + tk);
-		kerr_total = std::max(1, (int)(caps_.kerr_blocks * (1.0 - fg) + 0.5));
-		grid_total = std::max(1, (int)(caps_.grid_blocks * fg + 0.5));
+	/* FP64 lanes: enough for the list, on at most a quarter of the CUs (the
+	 * rest is carried over; FP64 rays are few, and each of its blocks
+	 * displaces Kerr ones) */
+	if (n_fb_ > 0)
+		p.fb_blocks = std::min(std::max(1, caps_.n_cu / 4), (n_fb_ + SRV_BLOCK - 1) / SRV_BLOCK);
+
+	/* grid blocks: beside Kerr work, its share of the GPU by the time each
+	 * kind's work left would take on all of it (units x steps per unit x
+	 * block-ms per step / blocks the GPU holds), at most one FP64 or grid
+	 * block per CU; alone, as many as its tiles fill */
+	if (tiles_left > 0) {
+		const int need = std::max(1, ((int)tiles_left + SRV_GWAVES - 1) / SRV_GWAVES);
+		int G;
+		if (wk_all > 0) {
+			const double tk = wk_all * spu(0) * bms_[0] / caps_.kerr_blocks;
+			const double tg = tiles_left * spu(1) * bms_[1] / caps_.grid_blocks;
+			const double fg = tg / (tg + tk);
+			const int room = std::max(1, caps_.n_cu - p.fb_blocks);
+			G = std::max(1, std::min(std::min(need, room), (int)(fg * caps_.grid_blocks + 0.5)));
+		} else {
+			G = std::min(need, std::max(1, caps_.grid_blocks - p.fb_blocks));
+		}
+		p.grid_blocks = G;
+		const int W = G * SRV_GWAVES;
+		p.g_out_cap = std::max(W, g_resume_);
+		size_t want_fresh = GRID_FRESH * (size_t)W;
+		want_fresh = want_fresh > (size_t)g_resume_ ? want_fresh - (size_t)g_resume_ : 0;
+		fill_segs(grid_reqs, want_fresh, GRID_CHUNK, (size_t)caps_.max_segs, p.gseg, &p.g_fresh);
 	}
 
-	/* Kerr: every class at least the blocks its resumed rays need (all of
-	 * them are claimed on the first refill), the rest by work, no more
-	 * than its work can fill */
+	/* Kerr: the room the FP64 and grid blocks leave (a CU holding one of
+	 * them keeps kerr_beside Kerr blocks, one holding two none), shared
+	 * among the classes by work, at least a block each */
 	if (n_act > 0) {
-		int b[SRV_MAX_CLASSES], want[SRV_MAX_CLASSES], used = 0;
+		const int heavy = p.fb_blocks + p.grid_blocks;
+		const int per_cu = std::max(1, caps_.kerr_blocks / std::max(1, caps_.n_cu));
+		int total;
+		if (heavy <= caps_.n_cu)
+			total = caps_.kerr_blocks - (per_cu - caps_.kerr_beside) * heavy;
+		else
+			total = caps_.kerr_beside * std::max(0, 2 * caps_.n_cu - heavy);
+		total = std::max(total, n_act);
+		int b[SRV_MAX_CLASSES], want[SRV_MAX_CLASSES];
 		for (int k = 0; k < n_act; ++k) {
-			const int c = act[k];
-			const int mn = std::max(1, (cls_[c].n_resume + SRV_BLOCK - 1) / SRV_BLOCK);
-			want[k] = std::max(mn, (int)((wk[c] + SRV_BLOCK - 1) / SRV_BLOCK));
-			b[k] = mn;
-			used += mn;
+			want[k] = std::max(1, (int)((wk[act[k]] + SRV_BLOCK - 1) / SRV_BLOCK));
+			b[k] = 1;
 		}
-		int rest = kerr_total - used;
+		int rest = total - n_act;
 		if (rest > 0) {
 			double w_open = 0.0;
 			for (int k = 0; k < n_act; ++k)
@@ -302,46 +330,43 @@ void Sched::plan_locked(Plan &p)
 			}
 		}
 		/* fresh rays: twice the class's lanes, within the FP64 list's
-		 * capacity (every ray of the epoch may be handed over once) */
+		 * capacity (every ray of the epoch may be handed over once) and
+		 * within the rays the GPU holds (the rays in flight after the
+		 * epoch, resumed or fresh, then fit the resume lists however the
+		 * classes' blocks change) */
 		size_t fb_room = caps_.fb_cap > n_fb_ ? (size_t)(caps_.fb_cap - n_fb_) : 0;
-		int block = 0;
+		size_t held = 0;
+		for (int k = 0; k < n_act; ++k)
+			held += (size_t)cls_[act[k]].n_resume;
+		const size_t lanes_all = (size_t)caps_.kerr_blocks * SRV_BLOCK;
+		size_t flight_room = lanes_all > held ? lanes_all - held : 0;
+		int block = 0, out = 0;
 		for (int k = 0; k < n_act; ++k) {
 			const int c = act[k];
 			ClassPlan &cp = p.cls[p.n_cls];
 			cp.key = cls_[c].key;
+			cp.slot = c;
 			cp.block0 = block;
 			cp.block1 = block + b[k];
 			block = cp.block1;
 			cp.res_off = cls_[c].res_off;
 			cp.n_resume = cls_[c].n_resume;
+			const int lanes = b[k] * SRV_BLOCK;
+			cp.out_base = out;
+			cp.out_cap = std::max(lanes, cp.n_resume);
+			out += cp.out_cap;
 			fb_room = fb_room > (size_t)cp.n_resume ? fb_room - (size_t)cp.n_resume : 0;
-			size_t want_fresh = 2 * (size_t)b[k] * SRV_BLOCK;
+			size_t want_fresh = 2 * (size_t)lanes;
 			want_fresh = want_fresh > (size_t)cp.n_resume ? want_fresh - (size_t)cp.n_resume : 0;
-			want_fresh = std::min(want_fresh, fb_room);
+			want_fresh = std::min(want_fresh, std::min(fb_room, flight_room));
 			cp.seg0 = (int)p.kseg.size();
-			unsigned fresh = 0;
-			fill_segs(members[c], want_fresh, KERR_CHUNK, (size_t)caps_.max_segs, p.kseg, &fresh);
+			fill_segs(members[c], want_fresh, KERR_CHUNK, (size_t)caps_.max_segs, p.kseg, &cp.n_fresh);
 			cp.seg1 = (int)p.kseg.size();
-			fb_room -= fresh;
-			cp.items = (unsigned)cp.n_resume + fresh;
-			cp.slot = c;
+			fb_room -= cp.n_fresh;
+			flight_room -= cp.n_fresh;
 			++p.n_cls;
 		}
 		p.kerr_blocks = block;
-	}
-
-	/* grid: at least a wave per resumed tile record, no more blocks than
-	 * the tiles can fill */
-	if (tiles_left > 0) {
-		const int mn = std::max(1, (g_resume_ + SRV_GWAVES - 1) / SRV_GWAVES);
-		const int want = std::max(mn, ((int)tiles_left + SRV_GWAVES - 1) / SRV_GWAVES);
-		p.grid_blocks = std::max(mn, std::min(want, grid_total));
-		p.g_resume = g_resume_;
-		size_t want_fresh = GRID_FRESH * (size_t)p.grid_blocks * SRV_GWAVES;
-		want_fresh = want_fresh > (size_t)g_resume_ ? want_fresh - (size_t)g_resume_ : 0;
-		unsigned fresh = 0;
-		fill_segs(grid_reqs, want_fresh, GRID_CHUNK, (size_t)caps_.max_segs, p.gseg, &fresh);
-		p.g_items = (unsigned)g_resume_ + fresh;
 	}
 }
 
@@ -356,32 +381,28 @@ void Sched::consume_locked(const Plan &p, const Result &res)
 		}
 	};
 	double fresh[2] = {0.0, 0.0};
+	for (int c = 0; c < SRV_MAX_CLASSES; ++c)
+		cls_[c].n_resume = 0;   /* a class with saved rays is always planned */
 	for (int k = 0; k < p.n_cls; ++k) {
 		const ClassPlan &cp = p.cls[k];
 		const int c = cp.slot;
-		const unsigned taken = std::min(res.claimed[k], cp.items);
-		advance(p.kseg, cp.seg0, cp.seg1, taken - (unsigned)cp.n_resume);
-		fresh[0] += taken - (unsigned)cp.n_resume;
-		cls_[c].n_resume = (int)res.saved[k];
-		cls_[c].res_off = cp.block0 * SRV_BLOCK;
+		const unsigned taken = std::min(res.fresh_claimed[k], cp.n_fresh);
+		advance(p.kseg, cp.seg0, cp.seg1, taken);
+		fresh[0] += taken;
+		cls_[c].n_resume = (int)std::min(res.saved[k], (unsigned)cp.out_cap);
+		cls_[c].res_off = cp.out_base + cp.out_cap - cls_[c].n_resume;
 	}
-	/* classes that sat out this epoch hold no saved rays */
-	for (int c = 0; c < SRV_MAX_CLASSES; ++c) {
-		bool in = false;
-		for (int k = 0; k < p.n_cls; ++k)
-			in = in || p.cls[k].slot == c;
-		if (!in)
-			cls_[c].n_resume = 0;
-	}
-	if (p.g_items > 0) {
-		const unsigned taken = std::min(res.g_claimed, p.g_items);
-		advance(p.gseg, 0, (int)p.gseg.size(), taken - (unsigned)p.g_resume);
-		fresh[1] += taken - (unsigned)p.g_resume;
-		g_resume_ = (int)res.g_saved;
+	if (p.grid_blocks > 0) {
+		const unsigned taken = std::min(res.g_fresh, p.g_fresh);
+		advance(p.gseg, 0, (int)p.gseg.size(), taken);
+		fresh[1] += taken;
+		g_resume_ = (int)std::min(res.g_saved, (unsigned)p.g_out_cap);
+		g_res_off_ = p.g_out_cap - g_resume_;
 	} else {
-		g_resume_ = 0;
+		g_resume_ = g_res_off_ = 0;
 	}
-	n_fb_ = (int)res.fb_out;
+	n_fb_ = (int)std::min(res.fb_saved, (unsigned)caps_.fb_cap);
+	fb_off_ = caps_.fb_cap - n_fb_;
 	flip_ ^= 1;
 	for (int tab : res.done) {
 		const int64_t id = tab_req_[(size_t)tab];
@@ -398,35 +419,22 @@ void Sched::consume_locked(const Plan &p, const Result &res)
 		steps_[k] += res.steps[k];
 	++epochs_;
 	/* work model: units started and steps taken per kind; block-ms per
-	 * step from epochs whose launch was busy throughout (it saved work) */
+	 * step from epochs whose launch was busy throughout (it saved work),
+	 * over the blocks that started before the stop time */
 	units_[0] += fresh[0];
 	units_[1] += fresh[1];
 	const double ksteps[2] = {(double)res.steps[0], (double)res.steps[2]};
 	const double kms[2] = {res.ms[0], res.ms[2]};
-	const int kblk[2] = {p.kerr_blocks, p.grid_blocks};
+	const double kblk[2] = {p.kerr_blocks - res.late[0] / 4.0, p.grid_blocks - res.late[2] / 4.0};
 	bool kfull[2] = {false, res.g_saved > 0};
 	for (int k = 0; k < p.n_cls; ++k)
 		kfull[0] = kfull[0] || res.saved[k] > 0;
 	for (int k = 0; k < 2; ++k)
-		if (kfull[k] && kms[k] > 0.0 && ksteps[k] > 0.0) {
+		if (kfull[k] && kms[k] > 0.0 && ksteps[k] > 0.0 && kblk[k] > 0.0) {
 			const double b = kms[k] * kblk[k] / ksteps[k];
 			bms_[k] = bms_seen_[k] ? 0.7 * bms_[k] + 0.3 * b : b;
 			bms_seen_[k] = true;
 		}
-	/* iterations per wave toward the target epoch time, from kernels that
-	 * ran them all (they saved work for the next epoch) */
-	if (cfg_.epoch_ms > 0) {
-		bool full[3] = {false, res.fb_out > 0, res.g_saved > 0};
-		for (int k = 0; k < p.n_cls; ++k)
-			full[0] = full[0] || res.saved[k] > 0;
-		for (int k = 0; k < 3; ++k) {
-			if (!full[k] || !(res.ms[k] > 0.0))
-				continue;
-			double f = cfg_.epoch_ms / res.ms[k];
-			f = std::min(2.0, std::max(0.5, f));
-			K_[k] = std::max(16, std::min(1 << 20, (int)(K_[k] * f)));
-		}
-	}
 }
 
 int Sched::step(int *busy, char *err, size_t errlen)
@@ -438,17 +446,19 @@ int Sched::step(int *busy, char *err, size_t errlen)
 		if (rc)
 			return rc;
 		plan_locked(p);
-		const bool work = p.n_cls > 0 || p.g_items > 0 || p.n_fb > 0;
+		const bool work = p.n_cls > 0 || p.grid_blocks > 0 || p.n_fb > 0;
 		if (busy)
 			*busy = work || !pending_.empty();
 		if (!work)
 			return GRAYHIP_OK;
 	}
 	Result res;
-	memset(res.claimed, 0, sizeof(res.claimed));
+	memset(res.res_claimed, 0, sizeof(res.res_claimed));
+	memset(res.fresh_claimed, 0, sizeof(res.fresh_claimed));
 	memset(res.saved, 0, sizeof(res.saved));
-	res.fb_out = res.g_claimed = res.g_saved = 0;
+	res.fb_saved = res.g_res = res.g_fresh = res.g_saved = res.overflow = 0;
 	res.steps[0] = res.steps[1] = res.steps[2] = 0;
+	res.late[0] = res.late[1] = res.late[2] = 0;
 	res.ms[0] = res.ms[1] = res.ms[2] = -1.0;
 	int rc = dev_->epoch(p, res);
 	std::lock_guard<std::mutex> lk(mu_);
@@ -456,18 +466,19 @@ int Sched::step(int *busy, char *err, size_t errlen)
 		set_err(err, errlen, dev_->error());
 		return rc;
 	}
-	/* every resumed ray and tile is claimed before any fresh one */
+	/* every resumed ray and tile is claimed or carried over, and no list
+	 * region overflowed */
 	for (int k = 0; k < p.n_cls; ++k)
-		if (res.claimed[k] < (unsigned)p.cls[k].n_resume) {
-			set_err(err, errlen, "epoch left resumed rays unclaimed");
+		if (res.res_claimed[k] < (unsigned)p.cls[k].n_resume) {
+			set_err(err, errlen, "epoch left resumed rays behind");
 			return GRAYHIP_EDEVICE;
 		}
-	if (p.g_items > 0 && res.g_claimed < (unsigned)p.g_resume) {
-		set_err(err, errlen, "epoch left resumed tiles unclaimed");
+	if (p.grid_blocks > 0 && res.g_res < (unsigned)p.g_resume) {
+		set_err(err, errlen, "epoch left resumed tiles behind");
 		return GRAYHIP_EDEVICE;
 	}
-	if (res.fb_out > (unsigned)caps_.fb_cap) {
-		set_err(err, errlen, "FP64 list overflowed");
+	if (res.overflow > 0) {
+		set_err(err, errlen, "a resume or FP64 list region overflowed");
 		return GRAYHIP_EDEVICE;
 	}
 	consume_locked(p, res);

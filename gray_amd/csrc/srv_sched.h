@@ -68,38 +68,50 @@ struct ClassPlan {
 	ClassKey key;
 	int slot;             /* scheduler's class index                        */
 	int block0, block1;   /* blocks of the Kerr launch                      */
-	int res_off, n_resume;/* resumed rays in the resume-in list             */
+	int res_off, n_resume;/* resumed rays in the resume in-list             */
+	int out_base, out_cap;/* the class's region of the resume out-list      */
 	int seg0, seg1;       /* its segments in Plan::kseg                     */
-	unsigned items;       /* n_resume + fresh rays                          */
+	unsigned n_fresh;     /* fresh rays in them                             */
 };
 
-/* One epoch's work for the device. */
+/* One epoch's work for the device.  The FP64 and grid launches start
+ * first; the Kerr launch fills the room they leave.  Every list an epoch
+ * reads is written anew into the other list of its pair (flip). */
 struct Plan {
 	int n_cls;
 	ClassPlan cls[SRV_MAX_CLASSES];
 	std::vector<Seg> kseg;
 	int kerr_blocks, K_kerr;
 	int grid_blocks, K_grid;
-	double grid_ms;       /* grid launch's time budget (<= 0: K_grid alone) */
-	int g_resume;         /* resumed tile records                          */
+	int g_res_off, g_resume;   /* resumed tile records in the in-list       */
+	int g_out_cap;             /* records of the out region                 */
 	std::vector<Seg> gseg;
-	unsigned g_items;
-	int n_fb, K_fb;       /* FP64 entries in the FP64-in list               */
-	int flip;             /* which of the two resume / FP64 lists is input  */
+	unsigned g_fresh;          /* fresh tiles in gseg                        */
+	int fb_blocks, K_fb;
+	int fb_off, n_fb;          /* FP64 entries in the FP64 in-list           */
+	double epoch_ms;           /* stop time of the launches (<= 0: their K)  */
+	int flip;                  /* which list of each pair is the in-list     */
 };
 
 struct Result {
-	unsigned claimed[SRV_MAX_CLASSES], saved[SRV_MAX_CLASSES];
-	unsigned fb_out, g_claimed, g_saved;
+	unsigned res_claimed[SRV_MAX_CLASSES], fresh_claimed[SRV_MAX_CLASSES];
+	unsigned saved[SRV_MAX_CLASSES];
+	unsigned fb_saved, g_res, g_fresh, g_saved, overflow;
 	std::vector<int> done;          /* table indices of finished requests    */
 	unsigned long long steps[3];
 	double ms[3];                   /* Kerr, FP64, grid kernel times (<0: none) */
+	unsigned late[3];               /* waves that started after the stop time */
 };
 
 /* Device limits the scheduler plans against. */
 struct Caps {
-	int kerr_blocks;      /* Kerr blocks of one launch                       */
-	int grid_blocks;      /* grid blocks of one launch (4 waves each)         */
+	int kerr_blocks;      /* Kerr blocks the GPU holds                        */
+	int grid_blocks;      /* grid blocks the GPU holds (4 waves each)          */
+	int n_cu;             /* compute units                                    */
+	int kerr_beside;      /* Kerr blocks a CU holds beside one FP64 or grid
+	                         block                                         */
+	int res_cap;          /* entries of each Kerr resume list                 */
+	int tile_cap;         /* records of each grid tile list                   */
 	int fb_cap;           /* entries of each FP64 list                        */
 	int max_segs;         /* fresh segments of one kind per epoch             */
 	int max_requests;     /* device table size                               */
@@ -146,7 +158,8 @@ private:
 	struct Class {
 		ClassKey key;
 		int refs;                 /* admitted unfinished requests           */
-		int n_resume, res_off;    /* rays saved at the last epoch           */
+		int n_resume, res_off;    /* rays saved at the last epoch (in the
+		                             resume list that is next read)        */
 	};
 
 	int admit_locked(char *err, size_t errlen);
@@ -167,8 +180,8 @@ private:
 	Arena arena_;
 	Class cls_[SRV_MAX_CLASSES];
 	bool cls_used_[SRV_MAX_CLASSES];
-	int g_resume_;                        /* grid tile records saved last epoch */
-	int n_fb_;                            /* FP64 entries for the next epoch   */
+	int g_res_off_, g_resume_;            /* grid records saved last epoch     */
+	int fb_off_, n_fb_;                   /* FP64 entries for the next epoch   */
 	int flip_;
 	int K_[3];                            /* Kerr, FP64, grid iterations       */
 	/* work model of the two launches (0 Kerr, 1 grid) */

@@ -5,13 +5,15 @@
  * tests/test_srv_sched_cpu.py.
  *
  * The mock follows the kernels' rules (kernels_serve.hip): a Kerr class's
- * lanes claim queue items in time order (resumed rays first), each lane
- * runs at most K iterations and saves the ray it still holds; a Kerr ray
- * may hand over to the FP64 list part-way; FP64 entries run at most K
- * steps; grid waves claim tiles and run each tile until its 64 rays are
- * done or K is used up.  Every ray takes a number of steps fixed by its
- * request tag and index, and its output row encodes tag, ray and steps,
- * so the checker knows every value in advance.
+ * lanes claim rays in time order (resumed rays first, then fresh ones),
+ * each lane runs at most K iterations and saves the ray it still holds; a
+ * Kerr ray may hand over to the FP64 list part-way; FP64 lanes claim list
+ * entries the same way; grid waves claim tiles (resumed records first) and
+ * run each tile until its 64 rays are done or K is used up.  Resumed
+ * entries nobody claimed are carried over unchanged, and every list is
+ * written from the back of its region.  Every ray takes a number of steps
+ * fixed by its request tag and index, and its output row encodes tag, ray
+ * and steps, so the checker knows every value in advance.
  *
  * Threads: one submits requests over time, one steps the pool, one
  * collects results (and polls state and stats) -- the concurrency the C
@@ -42,7 +44,7 @@ uint64_t mix64(uint64_t z)
 
 /* steps of ray g of the request tagged `tag`; a Kerr ray with bit 7 set
  * hands over to FP64 half-way */
-int64_t ray_steps(int tag, int g) { return 20 + (int64_t)(mix64((uint64_t)tag * 1000003u + (uint64_t)g) % 900); }
+int64_t ray_steps(int tag, int g) { return 20 + (int64_t)(mix64((uint64_t)tag * 1000003u + (uint64_t)g) % 2000); }
 bool ray_handover(int tag, int g) { return (mix64((uint64_t)tag * 7919u + (uint64_t)g) >> 7) & 1; }
 
 struct Ray {
@@ -71,8 +73,11 @@ public:
 	explicit Mock(const Caps &c) : caps(c), table((size_t)c.max_requests),
 	                              remaining((size_t)c.max_requests, 0), arena(c.arena / 8, 0.0)
 	{
-		for (int k = 0; k < 2; ++k)
-			res[k].resize((size_t)(c.kerr_blocks + SRV_MAX_CLASSES) * SRV_BLOCK);
+		for (int k = 0; k < 2; ++k) {
+			res[k].resize((size_t)c.res_cap);
+			fb[k].resize((size_t)c.fb_cap);
+			tiles[k].resize((size_t)c.tile_cap);
+		}
 	}
 	const char *error() const override { return err; }
 
@@ -106,36 +111,109 @@ public:
 
 	bool bad = false;
 
+	/* slot of the next entry written into region [base, base + cap), from
+	 * the back; -1 past its start */
+	int slot(unsigned &counter, int base, int cap)
+	{
+		const unsigned k = counter++;
+		if (k >= (unsigned)cap) {
+			++overflow;
+			return -1;
+		}
+		return base + cap - 1 - (int)k;
+	}
+
+	unsigned overflow = 0;
+	long carried[3] = {0, 0, 0};   /* entries carried over: Kerr, FP64, grid */
+
+	/* a lane pool: lanes claim items in time order; run(item, t) returns the
+	 * time the lane is free again, or a negative value once it saved */
+	template <class Claim, class Run>
+	static void pool(int lanes, int64_t K, Claim claim, Run run)
+	{
+		typedef std::pair<int64_t, int> Ev;
+		std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> pq;
+		for (int l = 0; l < lanes; ++l)
+			pq.push(Ev(0, l));
+		while (!pq.empty()) {
+			Ev e = pq.top();
+			pq.pop();
+			if (e.first >= K)
+				continue;            /* no refill once K is used up */
+			int item;
+			if (!claim(&item))
+				continue;            /* exhausted */
+			const int64_t t = run(item, e.first);
+			if (t >= 0)
+				pq.push(Ev(t, e.second));
+		}
+	}
+
 	int epoch(const Plan &p, Result &out) override
 	{
 		const int in = p.flip, o = p.flip ^ 1;
-		std::vector<Ray> fb_out;
-		std::vector<unsigned> saved(SRV_MAX_CLASSES, 0);
+		overflow = 0;
+		unsigned fb_saved = 0;
+		auto fb_put = [&](const Ray &y) {
+			const int e = slot(fb_saved, 0, caps.fb_cap);
+			if (e >= 0)
+				fb[o][(size_t)e] = y;
+		};
+		/* FP64 lanes */
+		{
+			unsigned cur = 0;
+			const int lanes = p.fb_blocks * SRV_BLOCK;
+			auto claim = [&](int *item) {
+				const unsigned k = cur++;
+				if (k >= (unsigned)p.n_fb)
+					return false;
+				*item = p.fb_off + (int)k;
+				return true;
+			};
+			auto run = [&](int item, int64_t t) -> int64_t {
+				Ray y = fb[in][(size_t)item];
+				const int64_t total = ray_steps(tag(y.tab), y.ray);
+				const int64_t need = total - y.done;
+				if (t + need <= p.K_fb) {
+					out.steps[1] += (unsigned long long)need;
+					y.done = total;
+					finish(y, out);
+					return t + need;
+				}
+				const int64_t k = p.K_fb - t;
+				out.steps[1] += (unsigned long long)k;
+				y.done += k;
+				fb_put(y);
+				return -1;
+			};
+			if (p.n_fb > 0 && lanes > 0)
+				pool(lanes, p.K_fb, claim, run);
+			/* entries nobody claimed are carried over */
+			for (unsigned k = cur; k < (unsigned)p.n_fb; ++k, ++carried[1])
+				fb_put(fb[in][(size_t)(p.fb_off + (int)k)]);
+		}
 		/* Kerr classes */
 		for (int k = 0; k < p.n_cls; ++k) {
 			const ClassPlan &c = p.cls[k];
 			const int lanes = (c.block1 - c.block0) * SRV_BLOCK;
-			unsigned next = 0, claims = 0;
-			/* lanes by the iteration at which they next claim */
-			typedef std::pair<int64_t, int> Ev;
-			std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> pq;
-			for (int l = 0; l < lanes; ++l)
-				pq.push(Ev(0, l));
-			while (!pq.empty()) {
-				Ev e = pq.top();
-				pq.pop();
-				const int64_t t = e.first;
-				if (t >= p.K_kerr)
-					continue;            /* no refill once K is used up */
-				++claims;
-				const unsigned idx = next++;
-				if (idx >= c.items)
-					continue;            /* exhausted */
+			unsigned rcur = 0, fcur = 0, saved = 0;
+			auto claim = [&](int *item) {
+				if (rcur < (unsigned)c.n_resume) {
+					*item = c.res_off + (int)rcur++;
+					return true;
+				}
+				const unsigned f = fcur++;
+				if (f >= c.n_fresh)
+					return false;
+				*item = -1 - (int)f;   /* fresh ray f */
+				return true;
+			};
+			auto run = [&](int item, int64_t t) -> int64_t {
 				Ray y;
-				if ((int)idx < c.n_resume) {
-					y = res[in][(size_t)(c.res_off + (int)idx)];
+				if (item >= 0) {
+					y = res[in][(size_t)item];
 				} else {
-					unsigned f = idx - (unsigned)c.n_resume;
+					unsigned f = (unsigned)(-1 - item);
 					int j = c.seg0;
 					while (f >= p.kseg[(size_t)j].count) {
 						f -= p.kseg[(size_t)j].count;
@@ -152,63 +230,52 @@ public:
 					y.done = stop;
 					if (stop < total) {
 						y.fb = true;
-						fb_out.push_back(y);
+						fb_put(y);
 					} else {
 						finish(y, out);
 					}
-					pq.push(Ev(t + need, e.second));
-				} else {
-					const int64_t run = p.K_kerr - t;
-					out.steps[0] += (unsigned long long)run;
-					y.done += run;
-					res[o][(size_t)(c.block0 * SRV_BLOCK) + saved[(size_t)k]++] = y;
+					return t + need;
 				}
+				const int64_t run_k = p.K_kerr - t;
+				out.steps[0] += (unsigned long long)run_k;
+				y.done += run_k;
+				const int e = slot(saved, c.out_base, c.out_cap);
+				if (e >= 0)
+					res[o][(size_t)e] = y;
+				return -1;
+			};
+			pool(lanes, p.K_kerr, claim, run);
+			/* resumed rays nobody claimed are carried over */
+			for (; rcur < (unsigned)c.n_resume; ++rcur, ++carried[0]) {
+				const int e = slot(saved, c.out_base, c.out_cap);
+				if (e >= 0)
+					res[o][(size_t)e] = res[in][(size_t)(c.res_off + (int)rcur)];
 			}
-			out.claimed[k] = claims;
-			out.saved[k] = saved[(size_t)k];
+			out.res_claimed[k] = rcur;
+			out.fresh_claimed[k] = fcur;
+			out.saved[k] = saved;
 		}
-		/* FP64 list */
-		for (int e = 0; e < p.n_fb; ++e) {
-			Ray y = fb[in][(size_t)e];
-			const int64_t total = ray_steps(tag(y.tab), y.ray);
-			const int64_t run = std::min<int64_t>(p.K_fb, total - y.done);
-			out.steps[1] += (unsigned long long)run;
-			y.done += run;
-			if (y.done >= total)
-				finish(y, out);
-			else
-				fb_out.push_back(y);
-		}
-		if ((int)fb_out.size() > caps.fb_cap) {
-			snprintf(err, sizeof(err), "FP64 list beyond its capacity");
-			return GRAYHIP_EDEVICE;
-		}
-		fb[o] = fb_out;
-		out.fb_out = (unsigned)fb_out.size();
 		/* grid waves */
-		tiles[o].clear();
-		if (p.g_items > 0) {
+		if (p.grid_blocks > 0) {
 			const int waves = p.grid_blocks * SRV_GWAVES;
-			unsigned next = 0, claims = 0;
-			typedef std::pair<int64_t, int> Ev;
-			std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> pq;
-			for (int w = 0; w < waves; ++w)
-				pq.push(Ev(0, w));
-			while (!pq.empty()) {
-				Ev e = pq.top();
-				pq.pop();
-				int64_t t = e.first;
-				if (t >= p.K_grid)
-					continue;
-				++claims;
-				const unsigned idx = next++;
-				if (idx >= p.g_items)
-					continue;
+			unsigned rcur = 0, fcur = 0, saved = 0;
+			auto claim = [&](int *item) {
+				if (rcur < (unsigned)p.g_resume) {
+					*item = p.g_res_off + (int)rcur++;
+					return true;
+				}
+				const unsigned f = fcur++;
+				if (f >= p.g_fresh)
+					return false;
+				*item = -1 - (int)f;
+				return true;
+			};
+			auto run = [&](int item, int64_t t) -> int64_t {
 				Tile T;
-				if ((int)idx < p.g_resume) {
-					T = tiles[in][idx];
+				if (item >= 0) {
+					T = tiles[in][(size_t)item];
 				} else {
-					unsigned f = idx - (unsigned)p.g_resume;
+					unsigned f = (unsigned)(-1 - item);
 					size_t j = 0;
 					while (f >= p.gseg[j].count) {
 						f -= p.gseg[j].count;
@@ -229,14 +296,14 @@ public:
 					if (T.live[l])
 						longest = std::max(longest, ray_steps(tag(T.tab), T.lane[l].ray) -
 						                                T.lane[l].done);
-				const int64_t run = std::min<int64_t>(longest, p.K_grid - t);
+				const int64_t run_k = std::min<int64_t>(longest, p.K_grid - t);
 				bool any = false;
 				for (int l = 0; l < 64; ++l) {
 					if (!T.live[l])
 						continue;
 					Ray &y = T.lane[l];
 					const int64_t left = ray_steps(tag(T.tab), y.ray) - y.done;
-					const int64_t k2 = std::min(left, run);
+					const int64_t k2 = std::min(left, run_k);
 					out.steps[2] += (unsigned long long)k2;
 					y.done += k2;
 					if (k2 == left) {
@@ -246,14 +313,25 @@ public:
 						any = true;
 					}
 				}
-				if (any)
-					tiles[o].push_back(T);   /* the wave used its K on this tile */
-				else
-					pq.push(Ev(t + run, e.second));
+				if (!any)
+					return t + run_k;
+				const int e = slot(saved, 0, p.g_out_cap);   /* K used up */
+				if (e >= 0)
+					tiles[o][(size_t)e] = T;
+				return -1;
+			};
+			pool(waves, p.K_grid, claim, run);
+			for (; rcur < (unsigned)p.g_resume; ++rcur, ++carried[2]) {
+				const int e = slot(saved, 0, p.g_out_cap);
+				if (e >= 0)
+					tiles[o][(size_t)e] = tiles[in][(size_t)(p.g_res_off + (int)rcur)];
 			}
-			out.g_claimed = claims;
-			out.g_saved = (unsigned)tiles[o].size();
+			out.g_res = rcur;
+			out.g_fresh = fcur;
+			out.g_saved = saved;
 		}
+		out.fb_saved = fb_saved;
+		out.overflow = overflow;
 		out.ms[0] = out.ms[1] = out.ms[2] = -1.0;
 		++epochs;
 		return bad ? GRAYHIP_EDEVICE : 0;
@@ -320,15 +398,19 @@ bool check(const Item &it, const std::vector<double> &d, const std::vector<doubl
 int main()
 {
 	Caps caps;
-	caps.kerr_blocks = 3;
-	caps.grid_blocks = 2;
+	caps.kerr_blocks = 6;
+	caps.grid_blocks = 4;
+	caps.n_cu = 2;
+	caps.kerr_beside = 1;
+	caps.res_cap = 2 * (caps.kerr_blocks + SRV_MAX_CLASSES) * SRV_BLOCK;
+	caps.tile_cap = 2 * (caps.grid_blocks * SRV_GWAVES + SRV_GWAVES);
 	caps.fb_cap = 1 << 14;
 	caps.max_segs = 1 << 12;
 	caps.max_requests = 24;
 	caps.arena = 3u << 20;
 	Mock mock(caps);
 	Config cfg;
-	cfg.epoch_steps = 700;
+	cfg.epoch_steps = 240;
 	cfg.epoch_ms = 0.0;
 	Sched sched(cfg, caps, &mock);
 
@@ -449,6 +531,8 @@ int main()
 	       failures.load());
 	if (failures.load() || checked.load() != N || st3[1] == 0 || st3[2] == 0)
 		return 1;
+	printf("carried kerr %ld fp64 %ld grid %ld\n", mock.carried[0], mock.carried[1],
+	       mock.carried[2]);
 	printf("OK\n");
 	return 0;
 }

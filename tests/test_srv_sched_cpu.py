@@ -3,7 +3,8 @@ follows the kernels' queue rules (tests/native/srv_mock_main.cpp): 240
 requests of uneven size, Kerr and gridded, with FP64 hand-overs, submitted
 from one thread while a second steps the pool and a third collects and
 checks every result; more Kerr classes than fit at once, an output budget
-smaller than the stream, a device table smaller than the stream.  Built and
+smaller than the stream, a device table smaller than the stream, launches
+with fewer lanes than entries to resume (carried over).  Built and
 run plainly, under ThreadSanitizer and under AddressSanitizer with
 UndefinedBehaviorSanitizer."""
 import os
@@ -38,3 +39,7 @@ def test_scheduler_with_mock_device(tmp_path, variant):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK" in r.stdout and "WARNING: ThreadSanitizer" not in r.stderr, r.stdout + r.stderr
+    # the run carried unclaimed entries of every list over (launches with
+    # fewer lanes than entries to resume)
+    line = [l for l in r.stdout.splitlines() if l.startswith("carried")][0].split()
+    assert all(int(line[k]) > 0 for k in (2, 4, 6)), line
