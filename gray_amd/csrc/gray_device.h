@@ -9,6 +9,14 @@
  * correctly rounded IEEE operations, so the plain FP64 form reproduces a
  * correctly rounded CPU evaluation of the same sequence bit for bit.
  *
+ * The geodesic RHS (evaluated four times per step) fuses its multiply-adds
+ * explicitly, as an OpenCL compiler may under the default FP_CONTRACT ON
+ * (KS.cl counts them, "35 (-9) FLOPs"), by one fixed rule: in a sum taken
+ * left to right, a product added to the running value is fused into it
+ * (fmaT); where both operands of an addition are products, the left one is
+ * rounded and the right one fused.  The once-per-ray camera set-up, down(),
+ * getuu() and the step rule are evaluated unfused.
+ *
  * The float instantiation is compiled with contraction on and uses the
  * hardware reciprocal / square-root / exp2 instructions (about 1 ulp).
  */
@@ -42,6 +50,17 @@ template <class T>
 __device__ __forceinline__ T dot4(const V4<T> &a, const V4<T> &b)
 {
 	return ((a.t * b.t + a.x * b.x) + a.y * b.y) + a.z * b.z;
+}
+
+/* a b + c with one rounding (v_fma_f64 / v_fma_f32). */
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+/* dot4 with every product after the first fused into the running sum. */
+template <class T>
+__device__ __forceinline__ T dot4f(const V4<T> &a, const V4<T> &b)
+{
+	return fmaT(a.z, b.z, fmaT(a.y, b.y, fmaT(a.x, b.x, a.t * b.t)));
 }
 
 /* sim-org/KS.cl:48-55 */
@@ -146,23 +165,23 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 	{
 		T aa = a * a;
 		T zz = q.z * q.z;
-		T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+		T kk = T(0.5) * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
 		T dd, rr, r, ir, half_rdd, two_rq;
 		if constexpr (sizeof(T) == 4) {
 			/* FP32 form: the same quantities from two reciprocal square
 			 * roots and one reciprocal (transcendentals issue at a quarter
 			 * of the VALU rate) instead of two square roots and four
 			 * reciprocals; 2 / (rr + aa zz / rr) = 2 rr / (rr^2 + aa zz). */
-			const T d2 = kk * kk + aa * zz;
+			const T d2 = fmaT(aa, zz, kk * kk);
 			const T idd = __builtin_amdgcn_rsqf(d2);
 			dd = d2 * idd;
 			rr = dd + kk;
 			ir = __builtin_amdgcn_rsqf(rr);
 			r = rr * ir;
 			half_rdd = (T(0.5) * ir) * idd;
-			two_rq = M::div(T(2.0) * rr, rr * rr + aa * zz);
+			two_rq = M::div(T(2.0) * rr, fmaT(aa, zz, rr * rr));
 		} else {
-			dd = M::sqrt_(kk * kk + aa * zz);
+			dd = M::sqrt_(fmaT(aa, zz, kk * kk));
 			rr = dd + kk;
 			r  = M::sqrt_(rr);
 			ir = M::div(T(1.0), r);
@@ -184,40 +203,40 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 		dx_r = tmp * q.x;
 		tmp  = two_rq;
 
-		tmp2 = T(3.0) - (T(2.0) * rr) * tmp;
+		tmp2 = fmaT(-(T(2.0) * rr), tmp, T(3.0));
 		f    = tmp * r;
 		dx_f = (tmp * dx_r) * tmp2;
 		dy_f = (tmp * dy_r) * tmp2;
-		dz_f = tmp * (dz_r * tmp2 - ((tmp * aa) * q.z) * ir);
+		dz_f = tmp * fmaT(-((tmp * aa) * q.z), ir, dz_r * tmp2);
 
 		T m2r  = T(-2.0) * r;
 		T issr = iss * r;
 		T issa = iss * a;
 
-		lx    = iss * (q.x * r + q.y * a);
-		tmp   = iss * (q.x + m2r * lx);
-		dx_lx = tmp * dx_r + issr;
-		dy_lx = tmp * dy_r + issa;
+		lx    = iss * fmaT(q.y, a, q.x * r);
+		tmp   = iss * fmaT(m2r, lx, q.x);
+		dx_lx = fmaT(tmp, dx_r, issr);
+		dy_lx = fmaT(tmp, dy_r, issa);
 		dz_lx = tmp * dz_r;
 
-		ly    = iss * (q.y * r - q.x * a);
-		tmp   = iss * (q.y + m2r * ly);
-		dx_ly = tmp * dx_r - issa;
-		dy_ly = tmp * dy_r + issr;
+		ly    = iss * fmaT(-q.x, a, q.y * r);
+		tmp   = iss * fmaT(m2r, ly, q.y);
+		dx_ly = fmaT(tmp, dx_r, -issa);
+		dy_ly = fmaT(tmp, dy_r, issr);
 		dz_ly = tmp * dz_r;
 
 		lz    = q.z * ir;
 		tmp   = (-lz) * ir;
 		dx_lz = tmp * dx_r;
 		dy_lz = tmp * dy_r;
-		dz_lz = tmp * dz_r + ir;
+		dz_lz = fmaT(tmp, dz_r, ir);
 	}
 
 	T hDxu, hDyu, hDzu;
 	V4<T> uD;
 	{
 		V4<T> Dx, Dy, Dz;
-		T lu  = ((u.t + lx * u.x) + ly * u.y) + lz * u.z;
+		T lu  = fmaT(lz, u.z, fmaT(ly, u.y, fmaT(lx, u.x, u.t)));
 		if (aux) {
 			aux->f = f;
 			aux->lx = lx;
@@ -226,36 +245,35 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 			aux->lu = lu;
 		}
 		T flu = f * lu;
-		Dx.t = dx_f * lu + f * ((dx_lx * u.x + dx_ly * u.y) + dx_lz * u.z);
-		Dy.t = dy_f * lu + f * ((dy_lx * u.x + dy_ly * u.y) + dy_lz * u.z);
-		Dz.t = dz_f * lu + f * ((dz_lx * u.x + dz_ly * u.y) + dz_lz * u.z);
+		Dx.t = fmaT(f, fmaT(dx_lz, u.z, fmaT(dx_ly, u.y, dx_lx * u.x)), dx_f * lu);
+		Dy.t = fmaT(f, fmaT(dy_lz, u.z, fmaT(dy_ly, u.y, dy_lx * u.x)), dy_f * lu);
+		Dz.t = fmaT(f, fmaT(dz_lz, u.z, fmaT(dz_ly, u.y, dz_lx * u.x)), dz_f * lu);
 
-		Dx.x = Dx.t * lx + flu * dx_lx;
-		Dx.y = Dx.t * ly + flu * dx_ly;
-		Dx.z = Dx.t * lz + flu * dx_lz;
+		Dx.x = fmaT(flu, dx_lx, Dx.t * lx);
+		Dx.y = fmaT(flu, dx_ly, Dx.t * ly);
+		Dx.z = fmaT(flu, dx_lz, Dx.t * lz);
 
-		Dy.x = Dy.t * lx + flu * dy_lx;
-		Dy.y = Dy.t * ly + flu * dy_ly;
-		Dy.z = Dy.t * lz + flu * dy_lz;
+		Dy.x = fmaT(flu, dy_lx, Dy.t * lx);
+		Dy.y = fmaT(flu, dy_ly, Dy.t * ly);
+		Dy.z = fmaT(flu, dy_lz, Dy.t * lz);
 
-		Dz.x = Dz.t * lx + flu * dz_lx;
-		Dz.y = Dz.t * ly + flu * dz_ly;
-		Dz.z = Dz.t * lz + flu * dz_lz;
+		Dz.x = fmaT(flu, dz_lx, Dz.t * lx);
+		Dz.y = fmaT(flu, dz_ly, Dz.t * ly);
+		Dz.z = fmaT(flu, dz_lz, Dz.t * lz);
 
-		hDxu = T(0.5) * dot4(Dx, u);
-		hDyu = T(0.5) * dot4(Dy, u);
-		hDzu = T(0.5) * dot4(Dz, u);
+		hDxu = T(0.5) * dot4f(Dx, u);
+		hDyu = T(0.5) * dot4f(Dy, u);
+		hDzu = T(0.5) * dot4f(Dz, u);
 
-		uD.t = (u.x * Dx.t + u.y * Dy.t) + u.z * Dz.t;
-		uD.x = (u.x * Dx.x + u.y * Dy.x) + u.z * Dz.x;
-		uD.y = (u.x * Dx.y + u.y * Dy.y) + u.z * Dz.y;
-		uD.z = (u.x * Dx.z + u.y * Dy.z) + u.z * Dz.z;
+		uD.t = fmaT(u.z, Dz.t, fmaT(u.y, Dy.t, u.x * Dx.t));
+		uD.x = fmaT(u.z, Dz.x, fmaT(u.y, Dy.x, u.x * Dx.x));
+		uD.y = fmaT(u.z, Dz.y, fmaT(u.y, Dy.y, u.x * Dx.y));
+		uD.z = fmaT(u.z, Dz.z, fmaT(u.y, Dy.z, u.x * Dx.z));
 
-		tmp = f * ((((-uD.t) + lx * (uD.x - hDxu)) + ly * (uD.y - hDyu)) +
-		           lz * (uD.z - hDzu));
+		tmp = f * fmaT(lz, uD.z - hDzu, fmaT(ly, uD.y - hDyu, fmaT(lx, uD.x - hDxu, -uD.t)));
 	}
-	V4<T> acc = {uD.t - tmp, (hDxu - uD.x) + lx * tmp,
-	             (hDyu - uD.y) + ly * tmp, (hDzu - uD.z) + lz * tmp};
+	V4<T> acc = {uD.t - tmp, fmaT(lx, tmp, hDxu - uD.x),
+	             fmaT(ly, tmp, hDyu - uD.y), fmaT(lz, tmp, hDzu - uD.z)};
 	return acc;
 }
 

@@ -32,7 +32,8 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 }
 
 /* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at
- * s + (dt/2) k1, s + (dt/2) k2, s + dt k3. */
+ * s + (dt/2) k1, s + (dt/2) k2, s + dt k3, each stage a fused multiply-add
+ * (as the geodesic RHS, gray_device.h); k1 + 2 (k2 + k3) likewise. */
 template <int NF, class P>
 __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, double *s, double dt)
 {
@@ -42,17 +43,17 @@ __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, dou
 
 	rhs_f64<NF, P>(p, rc, s, k1);
 #pragma unroll
-	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
+	for (int i = 0; i < ND; ++i) x[i] = fmaT(hdt, k1[i], s[i]);
 	rhs_f64<NF, P>(p, rc, x, k23);
 #pragma unroll
-	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
+	for (int i = 0; i < ND; ++i) x[i] = fmaT(hdt, k23[i], s[i]);
 	rhs_f64<NF, P>(p, rc, x, k);
 	/* k1 + 2 (k2 + k3) is formed before the last stage (same value), so
 	 * k1 and k2 + k3 are not both live across it */
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
-		k1[i] = k1[i] + 2.0 * (k23[i] + k[i]);
-		x[i] = s[i] + dt * k[i];
+		k1[i] = fmaT(2.0, k23[i] + k[i], k1[i]);
+		x[i] = fmaT(dt, k[i], s[i]);
 	}
 	rhs_f64<NF, P>(p, rc, x, k);
 #pragma unroll

@@ -2,8 +2,16 @@
  * gray_oracle.c -- FP64 CPU restatement of the GRay2 geodesic hot path.
  *
  * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Built with
- * -O2 -ffp-contract=off so that every expression below is evaluated exactly
+ * -ffp-contract=off so that every expression below is evaluated exactly
  * as parenthesised.  See gray_oracle.h for the conventions.
+ *
+ * The geodesic RHS and the RK4 stages fuse multiply-adds explicitly (FMA),
+ * as an OpenCL compiler may under the default FP_CONTRACT ON (KS.cl counts
+ * them), by one rule: in a sum taken left to right, a product added to the
+ * running value is fused; where both operands are products, the left one is
+ * rounded and the right one fused.  Built with -DOG_UNFUSED, FMA(a, b, c) is
+ * a * b + c with two roundings (liboracle_unfused.so), to show that the
+ * choice moves results far less than the parity tolerance.
  */
 #include "gray_oracle.h"
 
@@ -13,6 +21,12 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+#ifdef OG_UNFUSED
+#define FMA(a, b, c) ((a) * (b) + (c))
+#else
+#define FMA(a, b, c) fma((a), (b), (c))
 #endif
 
 /* OpenCL dot() restated left to right. */
@@ -25,6 +39,12 @@ static inline double dot3(double a0, double a1, double a2,
 static inline double dot4(const double a[4], const double b[4])
 {
 	return ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3];
+}
+
+/* dot4 with every product after the first fused into the running sum. */
+static inline double dot4f(const double a[4], const double b[4])
+{
+	return FMA(a[3], b[3], FMA(a[2], b[2], FMA(a[1], b[1], a[0] * b[0])));
 }
 
 void og_setup(og_params *p)
@@ -141,8 +161,8 @@ void og_gr_rhs(double a, const double q[4], const double u[4],
 	{
 		double aa = a * a;
 		double zz = q[3] * q[3];
-		double kk = 0.5 * (((q[1] * q[1] + q[2] * q[2]) + zz) - aa);
-		double dd = sqrt(kk * kk + aa * zz);
+		double kk = 0.5 * ((FMA(q[2], q[2], q[1] * q[1]) + zz) - aa);
+		double dd = sqrt(FMA(aa, zz, kk * kk));
 		double rr = dd + kk;
 		double r  = sqrt(rr);
 		double ir = 1.0 / r;
@@ -157,74 +177,73 @@ void og_gr_rhs(double a, const double q[4], const double u[4],
 		dx_r = tmp * q[1];
 		tmp  = 2.0 / (rr + (aa * zz) / rr);
 
-		tmp2 = 3.0 - (2.0 * rr) * tmp;
+		tmp2 = FMA(-(2.0 * rr), tmp, 3.0);
 		f    = tmp * r;
 		dx_f = (tmp * dx_r) * tmp2;
 		dy_f = (tmp * dy_r) * tmp2;
-		dz_f = tmp * (dz_r * tmp2 - ((tmp * aa) * q[3]) * ir);
+		dz_f = tmp * FMA(-((tmp * aa) * q[3]), ir, dz_r * tmp2);
 
 		double m2r  = -2.0 * r;
 		double issr = iss * r;
 		double issa = iss * a;
 
-		lx    = iss * (q[1] * r + q[2] * a);
-		tmp   = iss * (q[1] + m2r * lx);
-		dx_lx = tmp * dx_r + issr;
-		dy_lx = tmp * dy_r + issa;
+		lx    = iss * FMA(q[2], a, q[1] * r);
+		tmp   = iss * FMA(m2r, lx, q[1]);
+		dx_lx = FMA(tmp, dx_r, issr);
+		dy_lx = FMA(tmp, dy_r, issa);
 		dz_lx = tmp * dz_r;
 
-		ly    = iss * (q[2] * r - q[1] * a);
-		tmp   = iss * (q[2] + m2r * ly);
-		dx_ly = tmp * dx_r - issa;
-		dy_ly = tmp * dy_r + issr;
+		ly    = iss * FMA(-q[1], a, q[2] * r);
+		tmp   = iss * FMA(m2r, ly, q[2]);
+		dx_ly = FMA(tmp, dx_r, -issa);
+		dy_ly = FMA(tmp, dy_r, issr);
 		dz_ly = tmp * dz_r;
 
 		lz    = q[3] * ir;
 		tmp   = (-lz) * ir;
 		dx_lz = tmp * dx_r;
 		dy_lz = tmp * dy_r;
-		dz_lz = tmp * dz_r + ir;
+		dz_lz = FMA(tmp, dz_r, ir);
 	}
 
 	/* u_{b,a} contractions (KS.cl:337-366) */
 	double hDxu, hDyu, hDzu, uD[4];
 	{
 		double Dx[4], Dy[4], Dz[4];
-		double lu  = ((u[0] + lx * u[1]) + ly * u[2]) + lz * u[3];
+		double lu  = FMA(lz, u[3], FMA(ly, u[2], FMA(lx, u[1], u[0])));
 		double flu = f * lu;
-		Dx[0] = dx_f * lu + f * ((dx_lx * u[1] + dx_ly * u[2]) + dx_lz * u[3]);
-		Dy[0] = dy_f * lu + f * ((dy_lx * u[1] + dy_ly * u[2]) + dy_lz * u[3]);
-		Dz[0] = dz_f * lu + f * ((dz_lx * u[1] + dz_ly * u[2]) + dz_lz * u[3]);
+		Dx[0] = FMA(f, FMA(dx_lz, u[3], FMA(dx_ly, u[2], dx_lx * u[1])), dx_f * lu);
+		Dy[0] = FMA(f, FMA(dy_lz, u[3], FMA(dy_ly, u[2], dy_lx * u[1])), dy_f * lu);
+		Dz[0] = FMA(f, FMA(dz_lz, u[3], FMA(dz_ly, u[2], dz_lx * u[1])), dz_f * lu);
 
-		Dx[1] = Dx[0] * lx + flu * dx_lx;
-		Dx[2] = Dx[0] * ly + flu * dx_ly;
-		Dx[3] = Dx[0] * lz + flu * dx_lz;
+		Dx[1] = FMA(flu, dx_lx, Dx[0] * lx);
+		Dx[2] = FMA(flu, dx_ly, Dx[0] * ly);
+		Dx[3] = FMA(flu, dx_lz, Dx[0] * lz);
 
-		Dy[1] = Dy[0] * lx + flu * dy_lx;
-		Dy[2] = Dy[0] * ly + flu * dy_ly;
-		Dy[3] = Dy[0] * lz + flu * dy_lz;
+		Dy[1] = FMA(flu, dy_lx, Dy[0] * lx);
+		Dy[2] = FMA(flu, dy_ly, Dy[0] * ly);
+		Dy[3] = FMA(flu, dy_lz, Dy[0] * lz);
 
-		Dz[1] = Dz[0] * lx + flu * dz_lx;
-		Dz[2] = Dz[0] * ly + flu * dz_ly;
-		Dz[3] = Dz[0] * lz + flu * dz_lz;
+		Dz[1] = FMA(flu, dz_lx, Dz[0] * lx);
+		Dz[2] = FMA(flu, dz_ly, Dz[0] * ly);
+		Dz[3] = FMA(flu, dz_lz, Dz[0] * lz);
 
-		hDxu = 0.5 * dot4(Dx, u);
-		hDyu = 0.5 * dot4(Dy, u);
-		hDzu = 0.5 * dot4(Dz, u);
+		hDxu = 0.5 * dot4f(Dx, u);
+		hDyu = 0.5 * dot4f(Dy, u);
+		hDzu = 0.5 * dot4f(Dz, u);
 
 		for (int i = 0; i < 4; ++i)
-			uD[i] = (u[1] * Dx[i] + u[2] * Dy[i]) + u[3] * Dz[i];
+			uD[i] = FMA(u[3], Dz[i], FMA(u[2], Dy[i], u[1] * Dx[i]));
 
-		tmp = f * ((((-uD[0]) + lx * (uD[1] - hDxu)) + ly * (uD[2] - hDyu)) +
-		           lz * (uD[3] - hDzu));
+		tmp = f * FMA(lz, uD[3] - hDzu, FMA(ly, uD[2] - hDyu, FMA(lx, uD[1] - hDxu, -uD[0])));
 	}
 
 	/* KS.cl:368-376 */
 	dq[0] = u[0]; dq[1] = u[1]; dq[2] = u[2]; dq[3] = u[3];
 	du[0] = uD[0] - tmp;
-	du[1] = (hDxu - uD[1]) + lx * tmp;
-	du[2] = (hDyu - uD[2]) + ly * tmp;
-	du[3] = (hDzu - uD[3]) + lz * tmp;
+	du[1] = FMA(lx, tmp, hDxu - uD[1]);
+	du[2] = FMA(ly, tmp, hDyu - uD[2]);
+	du[3] = FMA(lz, tmp, hDzu - uD[3]);
 }
 
 /* sim-org/phys.cl:48-64.  r and eps are computed as the reference does
@@ -338,14 +357,14 @@ void og_integrate(const og_params *p, double *s, double dt)
 	double hdt = 0.5 * dt;
 
 	og_rhs(p, s, k1);
-	for (int i = 0; i < nd; ++i) x[i] = s[i] + hdt * k1[i];
+	for (int i = 0; i < nd; ++i) x[i] = FMA(hdt, k1[i], s[i]);
 	og_rhs(p, x, k2);
-	for (int i = 0; i < nd; ++i) x[i] = s[i] + hdt * k2[i];
+	for (int i = 0; i < nd; ++i) x[i] = FMA(hdt, k2[i], s[i]);
 	og_rhs(p, x, k3);
-	for (int i = 0; i < nd; ++i) x[i] = s[i] + dt * k3[i];
+	for (int i = 0; i < nd; ++i) x[i] = FMA(dt, k3[i], s[i]);
 	og_rhs(p, x, k4);
 	for (int i = 0; i < nd; ++i)
-		s[i] = s[i] + (dt * ((k1[i] + 2.0 * (k2[i] + k3[i])) + k4[i])) / 6.0;
+		s[i] = s[i] + (dt * (FMA(2.0, k2[i] + k3[i], k1[i]) + k4[i])) / 6.0;
 }
 
 /* sim-org/driver.cl:36-67 (pixel centre -> alpha, beta -> state). */

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke entry point as the
 checker and by the benchmark's CPU-baseline leg.  The product path never
 imports this module.
 """
+import contextlib
 import ctypes as C
 import math
 import os
@@ -11,7 +12,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# The oracle as built (geodesic RHS and RK4 stages with fused multiply-adds)
+# and the same sources built with every FMA as a multiply and an add.
+_LIBS = {"fused": os.path.join(_HERE, "liboracle.so"),
+         "unfused": os.path.join(_HERE, "liboracle_unfused.so")}
 
 MAX_FREQ = 8
 ESC_REF, ESC_OUTWARD = 0, 1
@@ -32,15 +36,28 @@ class Params(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
+_variant = "fused"
+
+
+@contextlib.contextmanager
+def unfused():
+    """Within this block every oracle call runs the unfused build."""
+    global _variant
+    old, _variant = _variant, "unfused"
+    try:
+        yield
+    finally:
+        _variant = old
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+    L = _libs.get(_variant)
+    if L is None:
+        path = _LIBS[_variant]
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         pp = C.POINTER(Params)
         L.og_setup.argtypes = [pp]
@@ -70,8 +87,8 @@ def lib():
         L.og_evolve_subset.argtypes = [pp, C.c_size_t, C.c_size_t, dp, C.c_double,
                                        C.c_size_t, C.c_size_t, C.c_int]
         L.og_evolve_subset.restype = C.c_int64
-        _lib = L
-    return _lib
+        _libs[_variant] = L
+    return L
 
 
 def _dp(x):
@@ -284,13 +301,12 @@ class Grid(C.Structure):
     ]
 
 
-_rt_set = False
+_rt_set = set()   # variants whose argtypes are set
 
 
 def _rlib():
-    global _rt_set
     L = lib()
-    if not _rt_set:
+    if _variant not in _rt_set:
         rp, dp, d = C.POINTER(RtConsts), C.POINTER(C.c_double), C.c_double
         L.og_log_k2.argtypes = [d]
         L.og_log_k2.restype = d
@@ -301,7 +317,7 @@ def _rlib():
             fn.argtypes = [rp] + [d] * n
             fn.restype = d
         L.og_rt_thermal.argtypes = [rp, C.c_int, dp, d, d, d, d, d, d, dp, dp, dp, dp]
-        _rt_set = True
+        _rt_set.add(_variant)
     return L
 
 
@@ -342,13 +358,12 @@ class Rt:
         return dI, dtau
 
 
-_grid_set = False
+_grid_set = set()   # variants whose argtypes are set
 
 
 def _glib():
-    global _grid_set
     L = lib()
-    if not _grid_set:
+    if _variant not in _grid_set:
         gp, dp = C.POINTER(Grid), C.POINTER(C.c_double)
         L.og_grid_down.argtypes = [gp, dp, dp, dp]
         L.og_grid_getuu.argtypes = [gp, dp, dp]
@@ -368,7 +383,7 @@ def _glib():
         L.og_grid_sample.restype = C.c_double
         L.og_grid_rhs_all.argtypes = [gp, dp, dp]
         L.og_grid_flow.argtypes = [gp, dp, dp, dp, dp]
-        _grid_set = True
+        _grid_set.add(_variant)
     return L
 
 

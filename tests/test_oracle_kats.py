@@ -259,3 +259,26 @@ def test_disk_flow_properties():
     # u^t of a prograde circular equatorial orbit (Bardeen et al. 1972)
     ut = (r ** 1.5 + a) / math.sqrt(r ** 0.75 * r ** 0.75 * (r ** 1.5 - 3 * r ** 0.5 + 2 * a))
     assert sh == pytest.approx(ut, rel=1e-12)
+
+
+def test_fused_and_unfused_oracles_agree():
+    """The oracle fuses the geodesic RHS's and RK4's multiply-adds (as an
+    OpenCL compiler may under FP_CONTRACT ON, sim-org/KS.cl:263-378,
+    RK4.cl:37-47).  The same sources with every FMA split into a multiply
+    and an add take the same steps and end within 1e-10 of each ray's state
+    scale on the config-1 camera (a = 0.9375, i = 60, 64 M plane), so the
+    choice is far inside the parity tolerance; the RHS itself differs by a
+    few ulp only."""
+    a = 0.9375
+    d0, _, s0 = O.render(O.make_params(a), 24, 24, 64.0, 64.0, 1000.0, 60.0, 0.0)
+    with O.unfused():
+        d1, _, s1 = O.render(O.make_params(a), 24, 24, 64.0, 64.0, 1000.0, 60.0, 0.0)
+        q = np.array([0.0, 3.1, -2.2, 1.7])
+        u = np.array([1.0, -0.3, 0.5, 0.2])
+        du1 = O.gr_rhs(a, q, u)[1]
+    du0 = O.gr_rhs(a, q, u)[1]
+    assert np.array_equal(s0, s1)
+    assert not np.array_equal(d0[:, :8], d1[:, :8])      # the two builds differ
+    scale = np.abs(d1[:, :8]).max(axis=1)
+    assert (np.abs(d0[:, :8] - d1[:, :8]).max(axis=1) / scale).max() <= 1e-10
+    assert np.abs(du0 - du1).max() <= 8 * np.finfo(float).eps * np.abs(du1).max()
