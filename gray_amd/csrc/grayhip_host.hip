@@ -69,10 +69,11 @@ static GridDesc grid_view(const grayhip_ctx *c);
 
 /* Constants of sim-org/rt.cl:28-143 in cgs (CODATA 2010 values, as there),
  * evaluated with the same expressions as the CPU oracle. */
-void gray::rt_thermal_init(RtThermal *r, double m_adm, int n_freq, const double *nu)
+void gray::rt_thermal_init(RtThermal *r, double m_adm, double b, int n_freq, const double *nu)
 {
 	const double c_light = 2.99792458e+10, h = 6.62606957e-27, G = 6.67384800e-08;
 	const double k_b = 1.38064881e-16, ry = 2.17987197e-11, m_e = 9.10938291e-28;
+	const double q_e = 4.80320425e-10;
 	const double m_sun = 1.98910000e+33, zeta = 1.78107241799;
 	const double pi = 3.14159265358979323846;
 	memset(r, 0, sizeof(*r));
@@ -94,6 +95,15 @@ void gray::rt_thermal_init(RtThermal *r, double m_adm, int n_freq, const double 
 	r->n_freq = n_freq;
 	for (int i = 0; i < n_freq; ++i)
 		r->nu[i] = nu[i];
+	/* synchrotron, rt.cl:145-170; T_e lies above the ln K_2 table, where
+	 * ln K_2 = ln(2 T_e^2 - 1/2) */
+	r->b = b;
+	r->te2b = (r->te * r->te) * b;
+	r->syn_nus = q_e / (((9.0 * pi) * m_e) * c_light);
+	r->syn_f = ((G * m_sun) / (c_light * c_light)) *
+	           ((sqrt(2.0) * pi) * (q_e * q_e) / (3.0 * c_light));
+	r->syn_k = pow(2.0, 11.0 / 12.0);
+	r->e_lk2 = exp(-log((2.0 * r->te) * r->te - 0.5));
 }
 
 /* Prograde ISCO (Bardeen, Press & Teukolsky 1972), M = 1. */
@@ -182,6 +192,11 @@ static int check_config(const grayhip_config *cfg)
 	for (int i = 0; i < cfg->n_freq; ++i)
 		if (!(cfg->nu[i] > 0.0))
 			return fail(nullptr, GRAYHIP_EINVAL, "frequency channel %d must be positive", i);
+	if (!(cfg->b_field >= 0.0 && cfg->b_field < 1e30))
+		return fail(nullptr, GRAYHIP_EINVAL, "b_field must be finite and >= 0");
+	if (cfg->b_field > 0.0 && !(cfg->fluid == GRAYHIP_FLUID_THERMAL && cfg->n_freq > 0))
+		return fail(nullptr, GRAYHIP_EINVAL, "b_field belongs to the thermal fluid's transfer "
+		                                     "(gridded spacetimes with channels)");
 
 	int ndev = grayhip_device_count();
 	if (ndev <= 0)
@@ -245,7 +260,7 @@ extern "C" int grayhip_create(const grayhip_config *cfg, grayhip_ctx **out)
 
 	fill_params(cfg, c->p);
 	c->st = cfg->spacetime;
-	rt_thermal_init(&c->rt, cfg->m_adm > 0.0 ? cfg->m_adm : 4.3e6,
+	rt_thermal_init(&c->rt, cfg->m_adm > 0.0 ? cfg->m_adm : 4.3e6, cfg->b_field,
 	                c->st ? cfg->n_freq : 0, cfg->nu);
 
 	int rc;

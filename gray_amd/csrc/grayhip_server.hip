@@ -249,7 +249,7 @@ public:
 		if (q->spacetime) {
 			D.gv = spacetime_view(q->spacetime, q->t_init, 0, q->escape == GRAYHIP_ESC_REF)
 			           .view<float>();
-			rt_thermal_init(&D.rt, 4.3e6, nf, q->nu);
+			rt_thermal_init(&D.rt, 4.3e6, 0.0, nf, q->nu);
 		}
 		D.kind = r.kind;
 		D.nd = r.nd;

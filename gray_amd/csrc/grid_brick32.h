@@ -162,15 +162,9 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 		ds[4 + mu] = -uGu[mu] + uGu[0] * u[mu];
 	if constexpr (NF > 0) {
 		if (nf > 0) {
-			float kt, ne;
-			lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
-			if (st.wt > 0.f) {
-				float kt2, ne2;
-				lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
-				kt = kt + st.wt * (kt2 - kt);
-				ne = ne + st.wt * (ne2 - ne);
-			}
-			thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF, nf);
+			float kt, ne, bkcos;
+			flow_f32<S>(G, rt, st, u, kt, ne, bkcos);
+			thermal_f32<NF>(rt, ne, -kt, bkcos, s + 8 + NF, ds + 8, ds + 8 + NF, nf);
 		} else {
 #pragma unroll
 			for (int i = 8; i < 8 + 2 * NF; ++i)

@@ -128,6 +128,59 @@ __device__ __forceinline__ void lev_flow(const S *__restrict__ met, const S *__r
 	}
 }
 
+/* lev_flow plus sum_k w_k (g_{z nu}(corner k) u^nu): k_z, from the z row
+ * {3, 6, 8, 9} of the metric, for the field's bkcos. */
+template <class S>
+__device__ __forceinline__ void lev_flow_z(const S *__restrict__ met, const S *__restrict__ rho,
+                                           const Stencil<float> &st, const float u[4], float &kt,
+                                           float &kz, float &ne)
+{
+	kt = 0.f;
+	kz = 0.f;
+	ne = 0.f;
+#pragma unroll 1
+	for (int k = 0; k < 8; ++k) {
+		const uint32_t p = st.corner(k);
+		float m[4], mz[4];
+		load4(gmem(met) + (size_t)p * 10, m);
+		load4(gmem(met) + (size_t)p * 10 + 6, mz);
+		const float w = st.weight(k);
+		kt += w * (((m[0] * u[0] + m[1] * u[1]) + m[2] * u[2]) + m[3] * u[3]);
+		kz += w * (((m[3] * u[0] + mz[0] * u[1]) + mz[2] * u[2]) + mz[3] * u[3]);
+		ne += w * (float)gmem(rho)[p];
+	}
+}
+
+/* k_t, n_e and bkcos (flow.cl:36-58, field b^mu = (0, 0, 0, B)) of both
+ * levels blended in time; bkcos = 0 without a field. */
+template <class S>
+__device__ __forceinline__ void flow_f32(const GridView<S> &G, const RtThermalF &rt,
+                                         const Stencil<float> &st, const float u[4], float &kt,
+                                         float &ne, float &bkcos)
+{
+	bkcos = 0.f;
+	if (rt.syn_on) {
+		float kz;
+		lev_flow_z<S>(G.met[0], G.rho[0], st, u, kt, kz, ne);
+		if (st.wt > 0.f) {
+			float kt2, kz2, ne2;
+			lev_flow_z<S>(G.met[1], G.rho[1], st, u, kt2, kz2, ne2);
+			kt = kt + st.wt * (kt2 - kt);
+			kz = kz + st.wt * (kz2 - kz);
+			ne = ne + st.wt * (ne2 - ne);
+		}
+		bkcos = (kz * rt.b) / ((-kt) * rt.b + 1e-28f);
+		return;
+	}
+	lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
+	if (st.wt > 0.f) {
+		float kt2, ne2;
+		lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
+		kt = kt + st.wt * (kt2 - kt);
+		ne = ne + st.wt * (ne2 - ne);
+	}
+}
+
 /* Geodesic RHS and, for NF > 0, the transfer of the reference flow on the
  * scaled state (I' = I / K, tau' = tau / K). */
 template <int NF, class S>
@@ -154,15 +207,9 @@ __device__ __forceinline__ void grid_rhs_all_f32(const GridView<S> &G, const RtT
 	for (int mu = 0; mu < 4; ++mu)
 		ds[4 + mu] = -uGu[mu] + uGu[0] * u[mu];
 	if constexpr (NF > 0) {
-		float kt, ne;
-		lev_flow<S>(G.met[0], G.rho[0], st, u, kt, ne);
-		if (st.wt > 0.f) {
-			float kt2, ne2;
-			lev_flow<S>(G.met[1], G.rho[1], st, u, kt2, ne2);
-			kt = kt + st.wt * (kt2 - kt);
-			ne = ne + st.wt * (ne2 - ne);
-		}
-		thermal_f32<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
+		float kt, ne, bkcos;
+		flow_f32<S>(G, rt, st, u, kt, ne, bkcos);
+		thermal_f32<NF>(rt, ne, -kt, bkcos, s + 8 + NF, ds + 8, ds + 8 + NF);
 	}
 }
 

@@ -151,7 +151,15 @@ __device__ __forceinline__ void grid_rhs_brick_f64(const GridView<S> &G, const R
 		sample_f64<4, 10, S>(G.met, 0, st, m);
 		sample_f64<1, 1, S>(G.rho, 0, st, &ne);
 		const double kt = ((m[0] * s[4] + m[1] * s[5]) + m[2] * s[6]) + m[3] * s[7];
-		thermal_f64<NF>(rt, ne, -kt, s + 8 + NF, ds + 8, ds + 8 + NF);
+		double bkcos = 0.0;
+		if (rt.b > 0.0) {
+			/* k_z from the z row {3, 6, 8, 9} of the metric; components 6..9 */
+			double mz[4];
+			sample_f64<4, 10, S>(G.met, 6, st, mz);
+			const double kz = ((m[3] * s[4] + mz[0] * s[5]) + mz[2] * s[6]) + mz[3] * s[7];
+			bkcos = (kz * rt.b) / ((-kt) * rt.b + RT_EPS);
+		}
+		thermal_f64<NF>(rt, ne, -kt, bkcos, s + 8 + NF, ds + 8, ds + 8 + NF);
 	}
 }
 

@@ -58,6 +58,7 @@ class Config(C.Structure):
         ("t_init", C.c_double),
         ("raw_icond", C.c_int),
         ("m_adm", C.c_double),
+        ("b_field", C.c_double),
     ]
 
 
@@ -199,7 +200,7 @@ class Render:
 
     def __init__(self, w_rays, h_rays, a_spin=0.0, precision=FP64, nu=(), fluid=None,
                  escape=ESC_OUTWARD, r_esc=1000.0, eps_stop=0.0, device=0,
-                 spacetime=None, t_init=0.0, raw_icond=False, m_adm=0.0):
+                 spacetime=None, t_init=0.0, raw_icond=False, m_adm=0.0, b_field=0.0):
         cfg = Config()
         cfg.w_rays, cfg.h_rays = w_rays, h_rays
         cfg.precision = precision
@@ -219,6 +220,7 @@ class Render:
         cfg.t_init = t_init
         cfg.raw_icond = int(raw_icond)
         cfg.m_adm = m_adm
+        cfg.b_field = b_field
         self._spacetime = spacetime  # keep it alive while the render exists
         self.cfg = cfg
         self.n_rays = w_rays * h_rays

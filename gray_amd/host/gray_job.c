@@ -144,6 +144,7 @@ static int _init(void *ego)
 		return EXIT_FAILURE;
 	}
 	cfg.m_adm = o->M_ADM;
+	cfg.b_field = o->b_field;
 	if (!EGO->n_freq)
 		cfg.fluid = GRAYHIP_FLUID_NONE;
 	if (!o->escape || !strcmp(o->escape, "outward"))

@@ -83,6 +83,8 @@ static const opt_row rows[] = {
 	{"fluid", T_STR, OFF(fluid), "", "build",
 	 "Fluid for radiative transfer: disk (analytic default), thermal (the reference flow "
 	 "on a numerical spacetime, its default), uniform or none"},
+	{"b_field:bz", T_REAL, OFF(b_field), "0", "build",
+	 "Field b^z of the thermal flow in Gauss (0: b = 0 as flow.cl; > 0 adds synchrotron)"},
 };
 
 #define N_ROWS (sizeof(rows) / sizeof(rows[0]))

@@ -44,6 +44,7 @@ struct gray_options {
 	double r_esc;
 	double eps_stop;            /* <= 0: rule default */
 	char  *fluid;               /* "disk" | "thermal" | "uniform" | "none"; "" by coordinates */
+	double b_field;             /* thermal flow's b^z in Gauss (0: the reference flow) */
 };
 
 /* Fill defaults (lux-optgen's X_init). */

@@ -56,10 +56,12 @@ enum {
 	GRAYHIP_FLUID_UNIFORM = 2,  /* n = 1, shift = 1 (slab closed forms)     */
 	GRAYHIP_FLUID_THERMAL = 3   /* the reference model, sim-org/flow.cl:36-58
 	                               with rt.cl:80-198: n_e from the
-	                               spacetime's density, T_e = 1e12, b = 0,
-	                               fluid at rest; bremsstrahlung emission
-	                               (synchrotron vanishes at b = 0) and
-	                               Kirchhoff absorption.  Gridded only.    */
+	                               spacetime's density, T_e = 1e12, fluid
+	                               at rest, field b^mu = (0,0,0,b_field)
+	                               (b = 0 in the reference); bremsstrahlung
+	                               and synchrotron emission (the latter
+	                               vanishes at b = 0), Kirchhoff
+	                               absorption.  Gridded only.              */
 };
 
 /* Grid coordinate maps of a numerical spacetime (sim-org/interp.cl:69-121
@@ -103,6 +105,9 @@ typedef struct {
 	int    raw_icond;        /* 1: no null normalisation (dyst.cl:89-110)   */
 	double m_adm;            /* M_ADM in solar masses for FLUID_THERMAL
 	                            (sim-org/param.opts:4); <= 0 picks 4.3e6    */
+	double b_field;          /* FLUID_THERMAL's field b^z in Gauss: 0 is the
+	                            reference flow (flow.cl:47, b = 0); > 0 adds
+	                            the synchrotron term of rt.cl:145-170     */
 } grayhip_config;
 
 typedef struct grayhip_ctx grayhip_ctx;

@@ -178,9 +178,12 @@ void og_grid_rhs(const og_grid *G, const double q[4], const double u[4], double 
 
 /* flow.cl:36-58: n_e = rho interpolated like every other field; the fluid
  * is at rest, u_fluid = (1,0,0,0), so shift = -k.u_fluid = -k_t with
- * k_t = g_{t nu} u^nu (the t row of dyst.cl:55-81). */
+ * k_t = g_{t nu} u^nu (the t row of dyst.cl:55-81).  The field 4-vector is
+ * b^mu = (0, 0, 0, B), B = b_field (flow.cl:47 sets b = 0, the default), so
+ * k.b = k_z B with k_z from the z row of the metric, and bkcos =
+ * k.b / (shift B + eps) (flow.cl:55); it is 0 for B = 0. */
 static void flow_at(const og_grid *G, const stencil *S, const double u[4], double *ne,
-                    double *shift)
+                    double *shift, double *bkcos)
 {
 	double m[4];
 	for (int c = 0; c < 4; ++c)
@@ -188,18 +191,26 @@ static void flow_at(const og_grid *G, const stencil *S, const double u[4], doubl
 	const double kt = ((m[0] * u[0] + m[1] * u[1]) + m[2] * u[2]) + m[3] * u[3];
 	*shift = -kt;
 	*ne = sample(G, G->rho, 1, 0, S);
+	*bkcos = 0.0;
+	if (G->b_field > 0.0) {
+		const double gxz = sample(G, G->met, 10, 6, S);
+		const double gyz = sample(G, G->met, 10, 8, S);
+		const double gzz = sample(G, G->met, 10, 9, S);
+		const double kz = ((m[3] * u[0] + gxz * u[1]) + gyz * u[2]) + gzz * u[3];
+		*bkcos = (kz * G->b_field) / (*shift * G->b_field + OG_RT_EPS);
+	}
 }
 
 void og_grid_flow(const og_grid *G, const double q[4], const double u[4], double *ne,
-                  double *shift)
+                  double *shift, double *bkcos)
 {
 	stencil S;
 	make_stencil(G, q, &S);
-	flow_at(G, &S, u, ne, shift);
+	flow_at(G, &S, u, ne, shift, bkcos);
 }
 
 /* phys.cl:66-74 on a gridded spacetime: the geodesic part, then rt_rhs of
- * rt.cl:185-198 with the flow above (T_e = 1e12, b = 0, bkcos = 0). */
+ * rt.cl:185-198 with the flow above (T_e = 1e12, B = b_field). */
 void og_grid_rhs_all(const og_grid *G, const double *s, double *ds)
 {
 	stencil S;
@@ -207,10 +218,10 @@ void og_grid_rhs_all(const og_grid *G, const double *s, double *ds)
 	ds[0] = s[4]; ds[1] = s[5]; ds[2] = s[6]; ds[3] = s[7];
 	rhs_at(G, &S, s + 4, ds + 4);
 	if (G->n_freq > 0) {
-		double ne, shift;
-		flow_at(G, &S, s + 4, &ne, &shift);
-		og_rt_thermal(G->rt, G->n_freq, G->nu, G->m_adm, ne, OG_RT_TE_FLOW, 0.0, 0.0, shift,
-		              s + 8, s + 8 + G->n_freq, ds + 8, ds + 8 + G->n_freq);
+		double ne, shift, bkcos;
+		flow_at(G, &S, s + 4, &ne, &shift, &bkcos);
+		og_rt_thermal(G->rt, G->n_freq, G->nu, G->m_adm, ne, OG_RT_TE_FLOW, G->b_field, bkcos,
+		              shift, s + 8, s + 8 + G->n_freq, ds + 8, ds + 8 + G->n_freq);
 	}
 }
 

@@ -38,12 +38,14 @@ typedef struct {
 	const double *gam[2];
 	const double *met[2];
 	/* radiative transfer with the reference flow (flow.cl:36-58): density
-	 * from the rho datasets, T_e = 1e12, b = 0, fluid at rest */
+	 * from the rho datasets, T_e = 1e12, fluid at rest, magnetic field
+	 * b^mu = (0, 0, 0, b_field) (flow.cl:47 has b = 0: b_field = 0) */
 	const double *rho[2];       /* per level, n_points; NULL without transfer */
 	int    n_freq;              /* channels; state = 8 + 2 n_freq reals      */
 	double nu[8];
 	double m_adm;               /* M_ADM in solar masses (param.opts:4)      */
 	const og_rt_consts *rt;
+	double b_field;             /* B in Gauss; 0: the reference flow          */
 } og_grid;
 
 static inline int og_grid_n_data(const og_grid *G) { return 8 + 2 * G->n_freq; }
@@ -58,7 +60,7 @@ double  og_grid_getdt(const og_grid *G, const double q[4], const double u[4], do
 void    og_grid_rhs_all(const og_grid *G, const double *s, double *ds);
 /* The flow at (q, u): density, and shift = -k.u_fluid with u_fluid at rest. */
 void    og_grid_flow(const og_grid *G, const double q[4], const double u[4], double *ne,
-                     double *shift);
+                     double *shift, double *bkcos);
 void    og_grid_integrate(const og_grid *G, double *s, double dt);
 int64_t og_grid_evolve_ray(const og_grid *G, double *s, double dt, size_t n_sub, int *done);
 void    og_grid_icond_all(const og_grid *G, size_t w_rays, size_t h_rays, double w_img,

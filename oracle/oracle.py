@@ -298,6 +298,7 @@ class Grid(C.Structure):
         ("nu", C.c_double * MAX_FREQ),
         ("m_adm", C.c_double),
         ("rt", C.POINTER(RtConsts)),
+        ("b_field", C.c_double),
     ]
 
 
@@ -382,7 +383,7 @@ def _glib():
         L.og_grid_sample.argtypes = [gp, C.c_int, C.c_int, dp]
         L.og_grid_sample.restype = C.c_double
         L.og_grid_rhs_all.argtypes = [gp, dp, dp]
-        L.og_grid_flow.argtypes = [gp, dp, dp, dp, dp]
+        L.og_grid_flow.argtypes = [gp, dp, dp, dp, dp, dp]
         _grid_set.add(_variant)
     return L
 
@@ -392,7 +393,8 @@ class GridSpacetime:
     gam[level] (n_points x 40) and met[level] (n_points x 10)."""
 
     def __init__(self, n, lo, hi, gam, met, times=(0.0, 0.0), map=MAP_UNIFORM, t_off=0.0,
-                 raw_icond=False, ref_rule=False, rho=None, nu=(), m_adm=M_ADM_DEFAULT):
+                 raw_icond=False, ref_rule=False, rho=None, nu=(), m_adm=M_ADM_DEFAULT,
+                 b_field=0.0):
         g = Grid()
         for a in range(3):
             g.n[a], g.lo[a], g.hi[a] = n[a], lo[a], hi[a]
@@ -417,6 +419,7 @@ class GridSpacetime:
         for i, v in enumerate(nu):
             g.nu[i] = v
         g.m_adm = m_adm
+        g.b_field = b_field
         self.rt = Rt()
         g.rt = C.pointer(self.rt.c)
         self.n_data = 8 + 2 * len(nu)
@@ -428,10 +431,12 @@ class GridSpacetime:
         _glib().og_grid_rhs_all(C.byref(self.g), _dp(s), _dp(ds))
         return ds
 
-    def flow(self, q, u):
-        ne, shift = C.c_double(), C.c_double()
+    def flow(self, q, u, with_bkcos=False):
+        ne, shift, bkcos = C.c_double(), C.c_double(), C.c_double()
         _glib().og_grid_flow(C.byref(self.g), _dp(_vec(q)), _dp(_vec(u)), C.byref(ne),
-                             C.byref(shift))
+                             C.byref(shift), C.byref(bkcos))
+        if with_bkcos:
+            return ne.value, shift.value, bkcos.value
         return ne.value, shift.value
 
     def sample(self, which, comp, q):

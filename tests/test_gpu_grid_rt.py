@@ -11,6 +11,10 @@ Bars:
     forms (FP64 1e-12, FP32 form 1e-5 relative);
   * FP32 form (FP32 grid, state scaled by K = M_ADM ff_f^2) against the FP64
     form: image relative L1 per channel <= 1e-4.
+The same bars hold with a magnetic field b^mu = (0, 0, 0, B) in the flow
+(flow.cl:47 has B = 0), which adds the synchrotron term of rt.cl:145-170;
+on the flat grid the closed form becomes I = (j_ff + j_syn(cos)) L with
+cos = k_z B / (shift B + eps) = u^z / u^t constant along each ray.
 """
 import os
 import subprocess
@@ -62,8 +66,9 @@ def device(n, lv, times, dtype, lo=-BOX, hi=BOX):
 
 
 def gpu_render(st, precision, w, nus=NUS, t_init=0.0, n_dump=2, cam=(32.0, 35.0, 45.0),
-               dt=-32.0, n_sub=1024):
-    with G.Render(w, w, precision=precision, spacetime=st, nu=nus, t_init=t_init) as r:
+               dt=-32.0, n_sub=1024, b_field=0.0):
+    with G.Render(w, w, precision=precision, spacetime=st, nu=nus, t_init=t_init,
+                  b_field=b_field) as r:
         r.icond(cam[0], cam[0], cam[1], cam[2], 0.0)
         for _ in range(n_dump):
             r.evolve(dt, n_sub)
@@ -169,3 +174,77 @@ def test_launcher_dyst_job_with_transfer(tmp_path):
     states = states.reshape(-1, 12)
     assert np.array_equal(states[:, :8], d0[:, :8])
     same_rt(d0[:, 8:], states[:, 8:])
+
+
+B_FIELD = 20.0
+
+
+def test_synchrotron_fp64_matches_oracle():
+    n, times = 33, (0.0, 1.0)
+    lv = boosted(n, times, 8)
+    st = device(n, lv, times, 8)
+    d1, i1, s1 = gpu_render(st, G.FP64, 12, t_init=1.0, b_field=B_FIELD)
+    d2, _, _ = gpu_render(st, G.FP64, 12, t_init=1.0)
+    st.close()
+    og = O.GridSpacetime((n,) * 3, (-BOX,) * 3, (BOX,) * 3, [l[0] for l in lv], [l[1] for l in lv],
+                         times=times, t_off=1.0, rho=[l[2] for l in lv], nu=NUS, b_field=B_FIELD)
+    d0, i0, s0 = og.render(12, 12, 32.0, 32.0, 35.0, 45.0, 0.0, n_dump=2)
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(d0[:, :8], d1[:, :8])
+    assert np.array_equal(i0, i1)
+    same_rt(d0[:, 8:], d1[:, 8:])
+    # the field's emission dominates the image: without it the intensities
+    # are those of bremsstrahlung alone, many orders of magnitude fainter
+    nf = len(NUS)
+    assert d1[:, 8:8 + nf].sum() > 1e6 * d2[:, 8:8 + nf].sum() > 0
+
+
+@pytest.mark.parametrize("precision,tol", [(G.FP64, 1e-12), (G.FP32, 1e-5)])
+def test_flat_slab_with_field_closed_form(precision, tol):
+    n, L = 9, 10.0
+    npts = n ** 3
+    gam = np.zeros((npts, 40))
+    met = np.zeros((npts, 10))
+    met[:, 0], met[:, 4], met[:, 7], met[:, 9] = -1.0, 1.0, 1.0, 1.0
+    rho = np.full(npts, 0.6)
+    dtype = 8 if precision == G.FP64 else 4
+    tp = np.float64 if dtype == 8 else np.float32
+    st = device(n, [(gam.astype(tp), met.astype(tp), rho.astype(tp))], (0.0,), dtype, -L, L)
+    # a camera's rays are parallel: two inclinations give two field angles
+    d = np.concatenate([gpu_render(st, precision, 8, cam=(4.0, 5.0, inc), dt=-4.0, n_sub=64,
+                                   n_dump=6, b_field=B_FIELD)[0] for inc in (45.0, 75.0)])
+    st.close()
+    rt = O.Rt()
+    path = -d[:, 0]
+    cos = d[:, 7] / d[:, 4]                  # k_z / -k_t in flat space
+    assert (path > 4.0).all() and (np.abs(cos) < 0.99).all() and np.ptp(cos) > 0.4
+    for i, v in enumerate(NUS):
+        js = np.array([rt.j_syn(v, 1e12, 0.6, B_FIELD, c) for c in cos])
+        j = rt.j_ff(v, 1e12, 0.6) + js
+        B = rt.b_planck(v, 1e12)
+        assert (js > 1e10 * rt.j_ff(v, 1e12, 0.6)).all()
+        np.testing.assert_allclose(d[:, 8 + i], j * path, rtol=tol)
+        np.testing.assert_allclose(d[:, 8 + len(NUS) + i], j / B * path, rtol=tol)
+
+
+def test_synchrotron_fp32_form_vs_fp64():
+    n, times = 33, (0.0,)
+    lv64 = boosted(n, times, 8)
+    lv32 = [tuple(a.astype(np.float32) for a in l) for l in lv64]
+    st64, st32 = device(n, lv64, times, 8), device(n, lv32, times, 4)
+    d64, _, _ = gpu_render(st64, G.FP64, 32, n_dump=3, b_field=B_FIELD)
+    d32, _, _ = gpu_render(st32, G.FP32, 32, n_dump=3, b_field=B_FIELD)
+    st64.close()
+    st32.close()
+    nf = len(NUS)
+    for blk in (slice(8, 8 + nf), slice(8 + nf, 8 + 2 * nf)):
+        a, b = d32[:, blk], d64[:, blk]
+        l1 = np.abs(a - b).sum(axis=0) / np.abs(b).sum(axis=0)
+        assert (l1 <= 1e-4).all(), l1
+
+
+def test_field_needs_the_thermal_flow():
+    with pytest.raises(G.GrayHipError):
+        G.Render(4, 4, 0.5, precision=G.FP64, nu=(1.0,), b_field=1.0).close()
+    with pytest.raises(G.GrayHipError):
+        G.Render(4, 4, 0.5, precision=G.FP64, b_field=-1.0).close()

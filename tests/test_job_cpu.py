@@ -21,6 +21,7 @@ DEFAULTS = [
     ("kflags", "kf", "-w"), ("outfile", "out", "%04d.raw"), ("t_init", "t0", 0.0),
     ("dt_dump", "dt", -32.0), ("i_init", "i0", 0), ("n_sub", "n", 1024),
     ("n_dump", "N", 64), ("m_bh", "M", 4e6), ("a_spin", "a", 0.0),
+    ("b_field", "bz", 0.0),    # build addition: the thermal flow's field b^z
 ]
 
 

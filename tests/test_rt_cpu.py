@@ -172,6 +172,37 @@ def test_flat_slab_closed_form():
         assert s[12 + i] == pytest.approx(j / B * L, rel=1e-12)
 
 
+def test_flat_slab_with_field_closed_form():
+    """A uniform field b^mu = (0, 0, 0, B) (flow.cl:47 with B instead of 0):
+    in flat space with the fluid at rest, bkcos = k_z B / (shift B + eps)
+    (flow.cl:55) is the cosine between the ray and z, constant along a
+    straight ray, so I = (j_ff + j_syn(cos)) L and tau = I / B_nu."""
+    nu = (1e11, 8e11)
+    B = 20.0
+    n, lo, hi, gam, met, rho = flat_grid()
+    G = O.GridSpacetime(n, lo, hi, [gam], [met], rho=[rho], nu=nu, b_field=B)
+    u = [1.0, 0.6, 0.0, -0.8]
+    ne, shift, bkcos = G.flow([0.0, 1.0, 2.0, 3.0], u, with_bkcos=True)
+    assert shift == pytest.approx(1.0, rel=1e-15) and bkcos == pytest.approx(-0.8, rel=1e-15)
+    assert G.flow([0.0, 1.0, 2.0, 3.0], [1.0, 0.0, 0.0, 1.0], with_bkcos=True)[2] == \
+        pytest.approx(1.0, rel=1e-15)
+    s0 = np.zeros(G.n_data)
+    s0[:8] = [0.0, 1.0, 2.0, 3.0] + u
+    (s,) = run_rays(G, [s0])
+    L = 8.0
+    for i, v in enumerate(nu):
+        js = RT.j_syn(v, 1e12, 0.6, B, -0.8)
+        j = RT.j_ff(v, 1e12, 0.6) + js
+        assert js > 1e10 * RT.j_ff(v, 1e12, 0.6)        # the field's emission dominates
+        assert s[8 + i] == pytest.approx(j * L, rel=1e-12)
+        assert s[10 + i] == pytest.approx(j / RT.b_planck(v, 1e12) * L, rel=1e-12)
+    # B = 0 is the reference flow: bremsstrahlung alone, bkcos = 0
+    G0 = O.GridSpacetime(n, lo, hi, [gam], [met], rho=[rho], nu=nu)
+    assert G0.flow([0.0, 1.0, 2.0, 3.0], u, with_bkcos=True)[2] == 0.0
+    (s,) = run_rays(G0, [s0])
+    assert s[8] == pytest.approx(RT.j_ff(nu[0], 1e12, 0.6) * L, rel=1e-12)
+
+
 def test_flat_density_gradient_quadratic_emission():
     nu = (2.67e11,)
     n, lo, hi, gam, met, rho = flat_grid(rho_fn=lambda X, Y, Z: 1.0 + 0.05 * X - 0.02 * Z)
