@@ -64,7 +64,7 @@ PEAK_HBM_GBPS = 8000.0
 # profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
 # profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
 # k_grid_tiles_f64<0, double>).
-TRAFFIC_POOL_PASS = (64.10 + 80.41) * 2**20
+TRAFFIC_POOL_PASS = (58.24 + 80.18) * 2**20
 TRAFFIC_GRID_TILES = (526.93 + 21.78) * 2**20
 TRAFFIC_GRID_TILES64 = (4695.65 + 22.24) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
@@ -81,6 +81,32 @@ PEAK_LDS_GBPS = 256 * 128 * 2.4
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+_FMA_PROBE = None
+
+
+def fma_probe():
+    """Attainable non-packed FMA rates of this GPU (gray-fma-probe: 8
+    independent v_fma chains per lane, 16 waves per SIMD), the ceiling of a
+    VALU kernel built without packed FP32; measured once per run."""
+    global _FMA_PROBE
+    if _FMA_PROBE is None:
+        import subprocess
+        exe = os.path.join(ROOT, "gray_amd", "gray-fma-probe")
+        try:
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
+            _FMA_PROBE = json.loads(r.stdout.strip().splitlines()[-1])
+        except (OSError, subprocess.SubprocessError, ValueError, IndexError) as e:
+            _FMA_PROBE = {"error": repr(e)}
+    return _FMA_PROBE
+
+
+def probe_fracs(achieved, key):
+    p = fma_probe()
+    if key not in p:
+        return {"fma_probe": p}
+    return {"fma_probe_tflops": p[key], "frac_of_fma_probe": achieved / p[key]}
 
 
 def render(G, precision, w=W, a=A_SPIN, inc=I_OBS):
@@ -141,6 +167,7 @@ def headline(G, args, out):
         "kernel_ms_is": "span from the first pool pass's start to the last pass's end "
                         "(the passes overlap on two streams); per-dispatch durations are in "
                         "the rocprofv3 kernel trace under profiles/",
+        **probe_fracs(achieved, "fp32_fma_tflops"),
     }
     return img32, info32, value
 
@@ -158,7 +185,9 @@ def fp64_section(G, img32, info32, value, out):
         "value": fp64_rate, "unit": "ray-steps/s", "render_ms": 1e3 * wall,
         "ray_steps": st64, "evolve_launches": nl,
         "roofline": {"bound": "valu", "kernel": "k_evolve_f64<4>", "achieved": a64,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS},
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS,
+                     "kernel_ms": kn64 * 1e-6, "kernel_ms_is": "sum of the evolve launches",
+                     **probe_fracs(a64, "fp64_fma_tflops")},
     }
     out["speedup_fp32_vs_fp64"] = value / fp64_rate
     out["image_rel_l1_vs_fp64"] = l1(img32, img64)
@@ -217,9 +246,11 @@ def grid_section(G, out):
                     "unique_model": "two time levels x n^3 x 40 Christoffels x sizeof(T) + "
                                     "ray state read and written"},
         }
-        # the FP32 kernel reads its corners from LDS bricks (LDS the busiest
-        # unit); the FP64 kernel's L2-miss traffic is its largest fraction
-        bound = "lds" if dtype == 4 else "hbm"
+        # Both kernels read their corners from LDS bricks and LDS is their
+        # busiest unit (FP32 0.59, FP64 0.46 plus bank conflicts on 23% of
+        # CU cycles; VALU issue 0.56 and 0.50).  HBM is idle: the L2 misses
+        # come to a few GB/s, well under 1% of 8 TB/s.
+        bound = "lds"
         roof.update({"bound": bound, **{k: roof[bound][k] for k in ("achieved", "peak", "unit", "frac")}})
         res[f"{'fp32' if dtype == 4 else 'fp64'}_grid"] = {
             "form": "fp32" if prec == G.FP32 else "fp64", "value": steps / wall,
