@@ -312,13 +312,16 @@ def serve_grids(G, o):
 
 def render_item(G, it, nf, precision, grids):
     """One request of the stream rendered alone (the request loop's
-    parameters, gray_serve.c fill_request); returns (data, ray-steps, wall s)."""
-    nus = NUS[:nf]
+    parameters and channel frequencies, gray_serve.c fill_request); returns
+    (data, ray-steps, wall s)."""
+    from gray_amd import serve as SV
     if it.grid >= 0:
+        nus = SV.GRID_NUS[:nf]
         st = grids[(it.grid, 4 if precision == G.FP32 else 8)]
         r = G.Render(it.side, it.side, precision=precision, spacetime=st, t_init=1.0, nu=nus)
         cam, sched = (32.0, 32.0, 35.0), (DT, N_SUB, 4)
     else:
+        nus = SV.KERR_NUS[:nf]
         r = G.Render(it.side, it.side, it.a_spin, precision=precision, nu=nus)
         cam, sched = (IMG, IMG, R_OBS), (DT, N_SUB, N_DUMP)
     t0 = time.perf_counter()
