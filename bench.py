@@ -205,7 +205,7 @@ def grid_section(G, out):
     n = 128
     xs = S.axis(-40.0, 40.0, n)
     b = S.bks(1.0, 0.7, 0.3, (1, 0, 0))
-    lv = [S.bks_sample(b, t, xs, xs, xs) for t in (0.0, 1.0)]
+    lv = [S.bks_sample(b, t, xs, xs, xs, rho=True) for t in (0.0, 1.0)]
     res = {"workload": "512x512 rays, r_obs=35, i=45, 32Mx32M plane, boosted Kerr-Schild "
                        "a=0.7 v=0.3 along x on 128^3 over [-40,40]^3, levels 0.0 and 1.0, "
                        "t_init=1.0, dt=-32 x 4 dumps, n_sub=1024, stop at box exit"}
@@ -213,7 +213,7 @@ def grid_section(G, out):
     for dtype, prec in ((4, G.FP32), (8, G.FP64)):
         tp = np.float32 if dtype == 4 else np.float64
         st = G.Spacetime((n,) * 3, (-40.0,) * 3, (40.0,) * 3, dtype=dtype)
-        for slot, (g, m) in enumerate(lv):
+        for slot, (g, m, _) in enumerate(lv):
             st.upload(slot, float(slot), g.astype(tp), m.astype(tp))
         st.set_frozen(False)
         with G.Render(512, 512, precision=prec, spacetime=st, t_init=1.0) as r:
@@ -265,7 +265,59 @@ def grid_section(G, out):
                            "p99_pos_err": float(np.percentile(dx, 99)),
                            "max_pos_err": float(dx.max())}
     res["speedup_fp32_vs_fp64"] = res["fp32_grid"]["value"] / res["fp64_grid"]["value"]
+    res["transfer"] = grid_transfer(G, lv)
     out["grid"] = res
+
+
+def grid_transfer(G, lv, nus=(2.3e11, 3.45e11, 5.0e11, 8.6e11)):
+    """The same rays and grid with radiative transfer: 4 channels of the
+    reference flow (thermal bremsstrahlung and Kirchhoff absorption on the
+    generator's density 1/|x|, capped at 100 as the tests do), FP32 form on
+    the FP32 grid against the plain FP64 form on the FP64 grid."""
+    n = 128
+    res = {"channels_hz": list(nus), "fluid": "thermal (flow.cl, b = 0)"}
+    imgs = {}
+    for dtype, prec in ((4, G.FP32), (8, G.FP64)):
+        tp = np.float32 if dtype == 4 else np.float64
+        st = G.Spacetime((n,) * 3, (-40.0,) * 3, (40.0,) * 3, dtype=dtype)
+        for slot, (g, m, rho) in enumerate(lv):
+            st.upload(slot, float(slot), g.astype(tp), m.astype(tp))
+            st.upload_rho(slot, np.minimum(rho, 100.0).astype(tp))
+        st.set_frozen(False)
+        with G.Render(512, 512, precision=prec, spacetime=st, t_init=1.0, nu=nus) as r:
+            t0 = time.perf_counter()
+            r.icond(32.0, 32.0, 35.0, 45.0, 0.0)
+            r.render(-32.0, 1024, 4)
+            wall = time.perf_counter() - t0
+            kn, _ = r.kernel_time()
+            steps = int(r.steps().sum())
+            dfin = r.read_f64()[0]
+            imgs[dtype] = dfin[:, 8:8 + len(nus)]
+            if dtype == 8:
+                # rays that end at the box wall (the others stop at the
+                # horizon proxy, where the static flow's shift -k_t can pass
+                # through zero and dI ~ 1/shift^2 has no scale)
+                leave = np.abs(dfin[:, 1:4]).max(axis=1) > 39.0
+        st.close()
+        res["fp32" if dtype == 4 else "fp64"] = {
+            "kernel": f"k_grid_tiles_{'f32' if dtype == 4 else 'f64'}<4, "
+                      f"{'float' if dtype == 4 else 'double'}>",
+            "value": steps / wall, "unit": "ray-steps/s", "ray_steps": steps,
+            "render_ms": 1e3 * wall, "kernel_ms": kn * 1e-6}
+    a, b = imgs[4], imgs[8]
+    fin = np.isfinite(a).all(1) & np.isfinite(b).all(1)
+    res["image_rel_l1_fp32_vs_fp64"] = (np.abs(a[fin] - b[fin]).sum(0) /
+                                        np.abs(b[fin]).sum(0)).tolist()
+    res["finite_share"] = float(fin.mean())
+    esc = fin & leave
+    res["image_rel_l1_leaving_rays"] = (np.abs(a[esc] - b[esc]).sum(0) /
+                                        np.abs(b[esc]).sum(0)).tolist()
+    res["leaving_share"] = float(leave.mean())
+    err = np.abs(a[fin, 0] - b[fin, 0])
+    top = np.sort(err)[::-1][:max(1, err.size // 1000)]
+    res["error_share_of_worst_0.1pct_rays"] = float(top.sum() / err.sum()) if err.sum() > 0 else 0.0
+    res["speedup_fp32_vs_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
+    return res
 
 
 # Stratified sample of the seeded 2000-request stream (seed 0x47524159):

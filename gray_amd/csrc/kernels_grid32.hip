@@ -46,7 +46,11 @@ constexpr int GP_WAVES = 4;                 /* waves per block             */
  * and advance at similar rates, so the rays in flight sweep the grid as one
  * front and the cells they read stay in L2. */
 template <int NF, class S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF == 0 ? 2 : 1)))
+/* Two waves per SIMD (as the LDS bricks allow) up to 5 channels, even where
+ * that spills (3 to 5 channels: 52-196 B/lane, 1.6x faster than one wave on
+ * 512^2 rays through the 128^3 grid, same bits); from 6 channels the spills
+ * grow past 1 KB/lane (3.3x slower at 8), and one wave holds the state. */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 5 ? 2 : 1)))
 void k_grid_tiles_f32(GridView<S> G, RtThermal rt64, StepRule sr, double *__restrict__ data,
                       int64_t *__restrict__ steps, size_t n_rays, int W, int H, int bricks,
                       unsigned long long *__restrict__ n_steps)
