@@ -535,19 +535,21 @@ static int run_fp32(grayhip_ctx *c, const StepRule &sr, double *ns)
 	return GRAYHIP_OK;
 }
 
-/* Plain FP64 form: one launch per interval. */
+/* Plain FP64 form: every interval in one launch (k_evolve_f64's flat loop;
+ * the same bits as one launch per interval). */
 static int run_fp64(grayhip_ctx *c, const StepRule &one, size_t n_dump, double *ns)
 {
+	StepRule sr = one;
+	sr.n_dump = (int)n_dump;
 	HCK(c, hipEventRecord(c->ev[0], c->stream));
-	for (size_t d = 0; d < n_dump; ++d)
-		HCK(c, launch_evolve_f64(c->p, one, c->d_data, c->d_info, c->d_steps, c->n_rays,
-		                         c->stream));
+	HCK(c, launch_evolve_f64(c->p, sr, c->d_data, c->d_info, c->d_steps, c->n_rays,
+	                         c->stream));
 	HCK(c, hipEventRecord(c->ev[1], c->stream));
 	HCK(c, hipStreamSynchronize(c->stream));
 	float ms = 0.f;
 	HCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
 	c->kern_ns = 1e6 * (double)ms;
-	c->kern_launches = (int)n_dump;
+	c->kern_launches = 1;
 	c->last_fallback = 0;
 	if (ns)
 		*ns = 1e6 * (double)ms;
@@ -845,9 +847,8 @@ extern "C" int grayhip_render_static(const grayhip_config *cfg, double w_img, do
 
 	SCKG(launch_icond(p, data, info, steps, n, st));
 	if (cfg->precision == GRAYHIP_FP64) {
-		sr.n_dump = 1;
-		for (size_t d = 0; d < n_dump; ++d)
-			SCKG(launch_evolve_f64(p, sr, data, info, steps, n, st));
+		sr.n_dump = (int)n_dump;
+		SCKG(launch_evolve_f64(p, sr, data, info, steps, n, st));
 		return GRAYHIP_OK;
 	}
 	/* one pool pass over every ray, then the FP64 re-integration of its

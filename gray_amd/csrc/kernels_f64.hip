@@ -1,6 +1,6 @@
 /*
  * kernels_f64.hip -- FP64 kernels: camera initial conditions and the plain
- * FP64 evolve (one lane per ray, the reference's per-launch structure).
+ * FP64 evolve (one lane per ray, any number of intervals per launch).
  *
  * Built with -ffp-contract=off: the double arithmetic follows the written
  * order exactly (see gray_device.h).
@@ -32,7 +32,17 @@ __global__ __launch_bounds__(256) void k_icond(RayParams p, double *__restrict__
 	steps[g] = 0;
 }
 
-/* Plain FP64 form: one lane per ray, one interval per launch. */
+/* Plain FP64 form: one lane per ray, sr.n_dump intervals per launch in one
+ * flat loop.  Each iteration takes one RK4 step of the lane's own
+ * (interval, unit) position with interval_f64's quantisation, so a ray on a
+ * long dependency chain near the photon orbit holds neither the other lanes
+ * of its wave nor the later intervals of other rays (a launch per interval
+ * would wait for it at every interval's end).  The results are those of
+ * sr.n_dump one-interval launches bit for bit: at each interval boundary
+ * the channels' shared depth takes the round trip of the stored form
+ * (tau_0 = d_0 T stored, T = tau_0 / d_0 loaded), and a retired ray, which
+ * a later launch would retire again at once, takes the remaining round
+ * trips without steps. */
 template <int NF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 4 ? 2 : 1))) void k_evolve_f64(RayParams p, StepRule sr,
                                                     double *__restrict__ data,
@@ -52,8 +62,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NF <= 4 ? 2
 		s[i] = data[(size_t)i * n_rays + g];
 	if constexpr (NF > 0)
 		s[8 + NF] = s[8 + NF] / rc.d[0];
-	bool retired;
-	int64_t st = interval_f64<NF>(p, rc, sr, s, 0, &retired);
+	int64_t st = 0;
+	int d = 0, h = 0;
+	while (d < sr.n_dump) {
+		V4<double> q = {s[0], s[1], s[2], s[3]};
+		V4<double> u = {s[4], s[5], s[6], s[7]};
+		double x = getdt(p, q, u, sr.dts) / sr.dt * (double)sr.n;
+		int dh = (x == x) ? (int)x : 0;
+		if (!dh) {
+			/* retired: every later interval would retire at its first
+			 * step rule again */
+			if constexpr (NF > 0) {
+				for (++d; d < sr.n_dump; ++d)
+					s[8 + NF] = (rc.d[0] * s[8 + NF]) / rc.d[0];
+			}
+			break;
+		}
+		if (dh > sr.n - h)
+			dh = sr.n - h;
+		rk4_f64<NF>(p, rc, s, (double)dh * sr.dt / (double)sr.n);
+		++st;
+		h += dh;
+		if (h >= sr.n) {
+			h = 0;
+			++d;
+			if constexpr (NF > 0) {
+				if (d < sr.n_dump)
+					s[8 + NF] = (rc.d[0] * s[8 + NF]) / rc.d[0];
+			}
+		}
+	}
 #pragma unroll
 	for (int i = 0; i < 8 + NF; ++i)
 		data[(size_t)i * n_rays + g] = s[i];
