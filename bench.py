@@ -630,8 +630,8 @@ def main():
                 serving_section(G, args, out["fp64"]["value"], "plain-FP64", out)
             else:
                 # without the FP64 run: the plain FP64 form's capacity on the
-                # headline workload as measured in full runs (1.43e10 ray-steps/s)
-                serving_section(G, args, 1.43e10, "plain-FP64 (earlier full run)", out)
+                # headline workload as measured in full runs (1.58e10 ray-steps/s)
+                serving_section(G, args, 1.58e10, "plain-FP64 (earlier full run)", out)
             log(f"serving: p50 {out['serving']['p50_ms']:.0f} ms, "
                 f"p99 {out['serving']['p99_ms']:.0f} ms")
         optional("serving", serve)

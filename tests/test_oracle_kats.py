@@ -266,7 +266,7 @@ def test_fused_and_unfused_oracles_agree():
     OpenCL compiler may under FP_CONTRACT ON, sim-org/KS.cl:263-378,
     RK4.cl:37-47).  The same sources with every FMA split into a multiply
     and an add take the same steps and end within 1e-10 of each ray's state
-    scale on the config-1 camera (a = 0.9375, i = 60, 64 M plane), so the
+    scale on a 64 M camera at r = 1000 (a = 0.9375, i = 60), so the
     choice is far inside the parity tolerance; the RHS itself differs by a
     few ulp only."""
     a = 0.9375
