@@ -102,11 +102,15 @@ def fma_probe():
     return _FMA_PROBE
 
 
-def probe_fracs(achieved, key):
+def probe_fracs(achieved, key, key_occ):
+    """Fractions of the probe's FMA rate at 16 waves per SIMD and at the
+    kernel's own occupancy."""
     p = fma_probe()
-    if key not in p:
+    if key not in p or key_occ not in p:
         return {"fma_probe": p}
-    return {"fma_probe_tflops": p[key], "frac_of_fma_probe": achieved / p[key]}
+    return {"fma_probe_tflops": p[key], "frac_of_fma_probe": achieved / p[key],
+            "fma_probe_tflops_at_occupancy": p[key_occ],
+            "frac_of_fma_probe_at_occupancy": achieved / p[key_occ]}
 
 
 def render(G, precision, w=W, a=A_SPIN, inc=I_OBS):
@@ -159,7 +163,6 @@ def headline(G, args, out):
         "bound": "valu", "kernel": "k_pool_f32<4>",
         "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "frac_of_unpacked_fp32": achieved / PEAK_FP64_TFLOPS,
         "traffic": TRAFFIC_POOL_PASS, "traffic_unit": "B per pass",
         "traffic_source": TRAFFIC_SOURCE,
         "flop_per_ray_step": FLOP_PER_STEP,
@@ -167,7 +170,7 @@ def headline(G, args, out):
         "kernel_ms_is": "span from the first pool pass's start to the last pass's end "
                         "(the passes overlap on two streams); per-dispatch durations are in "
                         "the rocprofv3 kernel trace under profiles/",
-        **probe_fracs(achieved, "fp32_fma_tflops"),
+        **probe_fracs(achieved, "fp32_fma_tflops", "fp32_fma_tflops_3_waves"),
     }
     return img32, info32, value
 
@@ -187,7 +190,7 @@ def fp64_section(G, img32, info32, value, out):
         "roofline": {"bound": "valu", "kernel": "k_evolve_f64<4>", "achieved": a64,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS,
                      "kernel_ms": kn64 * 1e-6, "kernel_ms_is": "sum of the evolve launches",
-                     **probe_fracs(a64, "fp64_fma_tflops")},
+                     **probe_fracs(a64, "fp64_fma_tflops", "fp64_fma_tflops_2_waves")},
     }
     out["speedup_fp32_vs_fp64"] = value / fp64_rate
     out["image_rel_l1_vs_fp64"] = l1(img32, img64)
@@ -611,6 +614,8 @@ def main():
     if not args.no_fp64 and value is not None:
         fp64_section(G, img32, info32, value, out)
         log(f"fp64: {out['fp64']['value']:.3e} ray-steps/s")
+    if not args.no_headline:
+        out["fma_probe"] = fma_probe()
     def optional(name, fn):
         # a failing side section is reported in the line instead of losing it
         try:
