@@ -33,6 +33,7 @@ struct BrickHdr {
  * unrolled, their brick offsets and weights formed once (upper indices
  * clamped as in the stencil: a zero stride means the upper corner is the
  * lower one).  Weights and sums are the stencil's, in its order. */
+template <bool PF>
 __device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, const Stencil<float> &st,
                                              const int o[3], const float P[10], float out[4])
 {
@@ -44,16 +45,36 @@ __device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, c
 #pragma unroll
 	for (int mu = 0; mu < 4; ++mu)
 		out[mu] = 0.f;
+	/* PF (the kernels without transfer channels, which have the registers):
+	 * corner k + 1's ten 16-byte reads are issued before corner k is
+	 * summed, so a wave keeps a corner of LDS latency in flight; the sums
+	 * are the same either way */
+	float4 cur[10], nxt[10];
+	if constexpr (PF) {
+#pragma unroll
+		for (int i = 0; i < 10; ++i)
+			cur[i] = p0[i];
+	}
 #pragma unroll
 	for (int k = 0; k < 8; ++k) {
 		const float4 *p = p0 + ((k & 1) ? ox : 0) + ((k & 2) ? oy : 0) + ((k & 4) ? oz : 0);
+		if constexpr (PF) {
+			if (k < 7) {
+				const int k1 = k + 1;
+				const float4 *pn = p0 + ((k1 & 1) ? ox : 0) + ((k1 & 2) ? oy : 0) +
+				                   ((k1 & 4) ? oz : 0);
+#pragma unroll
+				for (int i = 0; i < 10; ++i)
+					nxt[i] = pn[i];
+			}
+		}
 		const float w = (gx[k & 1] * gy[(k >> 1) & 1]) * gz[k >> 2];
 #pragma unroll
 		for (int hf = 0; hf < 2; ++hf) {
 			float v[20];
 #pragma unroll
 			for (int i = 0; i < 5; ++i) {
-				const float4 q = p[5 * hf + i];
+				const float4 q = PF ? cur[5 * hf + i] : p[5 * hf + i];
 				v[4 * i] = q.x;
 				v[4 * i + 1] = q.y;
 				v[4 * i + 2] = q.z;
@@ -67,6 +88,11 @@ __device__ __forceinline__ void lds_contract(const float4 *__restrict__ brick, c
 					tt += v[10 * m2 + c] * P[c];
 				out[2 * hf + m2] += w * tt;
 			}
+		}
+		if constexpr (PF) {
+#pragma unroll
+			for (int i = 0; i < 10; ++i)
+				cur[i] = nxt[i];
 		}
 	}
 }
@@ -145,7 +171,7 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 		}
 	}
 	if (inside)
-		lds_contract(brick, st, o, P, uGu);
+		lds_contract<NF == 0>(brick, st, o, P, uGu);
 	else
 		lev_contract<S>(G.gam[0], st, P, uGu);
 	if (st.wt > 0.f) {
