@@ -235,6 +235,18 @@ def icond_all(p, w_rays, h_rays, w_img, h_img, r_obs, i_obs, j_obs):
     return data, info
 
 
+def icond_pixels(p, w_rays, h_rays, w_img, h_img, r_obs, i_obs, j_obs, pixels):
+    """AoS state of the listed pixels (gi, gj) of a w_rays x h_rays image,
+    og_icond_ray per pixel (for images too large to set up whole)."""
+    si, ci, sj, cj = camera_trig(i_obs, j_obs)
+    data = np.zeros((len(pixels), n_data(p)))
+    for k, (gi, gj) in enumerate(pixels):
+        row = data[k]
+        lib().og_icond_ray(C.byref(p), w_rays, h_rays, w_img, h_img, r_obs, si, ci, sj, cj,
+                           gi, gj, _dp(row))
+    return data
+
+
 def evolve_all(p, data, info, dt, n_sub, steps=None, n_threads=0):
     """One launch of sim-org/driver.cl:70-108 over every ray (in place)."""
     assert data.flags.c_contiguous and data.dtype == np.float64

@@ -1,7 +1,8 @@
 """Edge cases of the analytic-Kerr path on the GPU against the oracle:
 degenerate and ragged image shapes, spin at 0, negative and near-extremal,
-pole-on, edge-on and south-pole cameras, the maximum of 8 channels, and a
-zoomed plane entirely inside the shadow.
+pole-on, edge-on and south-pole cameras, the maximum of 8 channels, a
+zoomed plane entirely inside the shadow, and an 8192^2 image whose state
+arrays pass 2^31 bytes.
 
 Bars as in test_gpu_kerr.py: the plain FP64 form equals the oracle bit for
 bit on geodesic state and step counts, transfer channels to 1e-10; the FP32
@@ -102,3 +103,34 @@ def test_restart_keeps_shared_depths_and_rejects_others():
     I0, I1 = ref[:, 8:11], got32[:, 8:11]
     l1 = np.abs(I1 - I0).sum(0) / np.abs(I0).sum(0)
     assert (l1 <= 1e-4).all(), l1
+
+
+def test_large_image_8192_squared():
+    """67,108,864 rays: the SoA state (8 reals a ray) spans 4.3 GB, so
+    device offsets pass 2^31 bytes.  Two intervals of 16 sub-steps (the
+    rays stay in the far field).  Plain FP64 form bit for bit the oracle on
+    a lattice of pixels up to the last ray; FP32 form within 1e-6 relative
+    of it on positions, with the same step counts."""
+    w = h = 8192
+    a = 0.9375
+    cols = [0, 1, 2047, 4096, 6143, 8190, 8191]
+    pix = [(gi, gj) for gj in cols for gi in cols]
+    idx = np.array([gj * w + gi for gi, gj in pix])
+    p = O.make_params(a)
+    ref = O.icond_pixels(p, w, h, 64.0, 64.0, 1000.0, 60.0, 0.0, pix)
+    ref_steps = O.evolve_subset(p, ref, 1, -32.0, 16, 2)
+    out = {}
+    for prec in (G.FP64, G.FP32):
+        with G.Render(w, h, a, precision=prec) as r:
+            r.icond(64.0, 64.0, 1000.0, 60.0, 0.0)
+            r.render(-32.0, 16, 2)
+            d, _ = r.read_f64()
+            out[prec] = (d[idx].copy(), r.steps()[idx].copy(), int(r.steps().sum()))
+            del d
+    d64, s64, tot64 = out[G.FP64]
+    d32, s32, tot32 = out[G.FP32]
+    assert s64.sum() == ref_steps and tot64 > 16 * w * h
+    assert np.array_equal(d64[:, :8], ref[:, :8])
+    assert np.array_equal(s32, s64) and tot32 == tot64
+    rel = np.abs(d32[:, 1:4] - d64[:, 1:4]).max(axis=1) / np.abs(d64[:, 1:4]).max(axis=1)
+    assert rel.max() <= 1e-6, rel.max()
