@@ -35,8 +35,9 @@ namespace gray {
 constexpr int GT_WAVES = 2;                 /* waves per block */
 
 /* sample_lev_f64 with the corners in the brick (same sums, same order).
- * Unrolled, so the LDS reads of later corners issue while earlier ones are
- * summed; the global-memory corner loop stays rolled (unrolled, its
+ * Unrolled, with corner k + 1's reads issued before corner k is summed, so
+ * a wave keeps a corner of LDS latency in flight (253 VGPRs, two waves per
+ * SIMD); the global-memory corner loop stays rolled (unrolled, its
  * registers cost more than its overlap gains). */
 template <int NC, class S>
 __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, int base,
@@ -45,21 +46,26 @@ __device__ __forceinline__ void sample_brick_f64(const S *__restrict__ brick, in
 {
 	const int p0 = (st.c[0] - o[0]) + 4 * (st.c[1] - o[1]) + 16 * (st.c[2] - o[2]);
 	const int ox = st.sx ? 1 : 0, oy = st.sy ? 4 : 0, oz = st.sz ? 16 : 0;
-	{
-		const S *p = brick + p0 * 40 + base;
-		const double w = st.weight(0);
+	S cur[NC], nxt[NC];
 #pragma unroll
-		for (int c = 0; c < NC; ++c)
-			out[c] = w * (double)p[c];
-	}
+	for (int c = 0; c < NC; ++c)
+		cur[c] = brick[p0 * 40 + base + c];
 #pragma unroll
-	for (int k = 1; k < 8; ++k) {
-		const int pt = p0 + ((k & 1) ? ox : 0) + ((k & 2) ? oy : 0) + ((k & 4) ? oz : 0);
-		const S *p = brick + pt * 40 + base;
+	for (int k = 0; k < 8; ++k) {
+		if (k < 7) {
+			const int k1 = k + 1;
+			const int pn = p0 + ((k1 & 1) ? ox : 0) + ((k1 & 2) ? oy : 0) + ((k1 & 4) ? oz : 0);
+#pragma unroll
+			for (int c = 0; c < NC; ++c)
+				nxt[c] = brick[pn * 40 + base + c];
+		}
 		const double w = st.weight(k);
 #pragma unroll
 		for (int c = 0; c < NC; ++c)
-			out[c] = out[c] + w * (double)p[c];
+			out[c] = k == 0 ? w * (double)cur[c] : out[c] + w * (double)cur[c];
+#pragma unroll
+		for (int c = 0; c < NC; ++c)
+			cur[c] = nxt[c];
 	}
 }
 
