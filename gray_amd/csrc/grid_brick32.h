@@ -172,6 +172,8 @@ __device__ __forceinline__ void grid_rhs_brick(const GridView<S> &G, const RtThe
 	}
 	if (inside)
 		lds_contract<NF == 0>(brick, st, o, P, uGu);
+	else if constexpr (NF == 0)
+		lev_contract_pf<S>(G.gam[0], st, P, uGu);
 	else
 		lev_contract<S>(G.gam[0], st, P, uGu);
 	if (st.wt > 0.f) {

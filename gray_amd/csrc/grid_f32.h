@@ -94,6 +94,60 @@ __device__ __forceinline__ void lev_contract(const S *__restrict__ lev, const St
 	}
 }
 
+/* lev_contract with the next half-corner's five 16-byte loads issued before
+ * the current half is summed (FP32 storage; the same sums in the same
+ * order); for the kernels that have the registers. */
+template <class S>
+__device__ __forceinline__ void lev_contract_pf(const S *__restrict__ lev, const Stencil<float> &st,
+                                                const float P[10], float out[4])
+{
+	if constexpr (sizeof(S) != 4) {
+		lev_contract<S>(lev, st, P, out);
+	} else {
+		gptr<f32x4> a4 = (gptr<f32x4>)gmem(lev);
+		/* sixteen half-corners of 20 values; half j + 1's five loads are
+		 * issued before half j is summed */
+		f32x4 cur[5], nxt[5];
+#pragma unroll
+		for (int i = 0; i < 5; ++i)
+			cur[i] = a4[(size_t)st.corner(0) * 10 + i];
+#pragma unroll
+		for (int mu = 0; mu < 4; ++mu)
+			out[mu] = 0.f;
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			const int k = j >> 1, hf = j & 1;
+			if (j < 15) {
+				const int k1 = (j + 1) >> 1, h1 = (j + 1) & 1;
+#pragma unroll
+				for (int i = 0; i < 5; ++i)
+					nxt[i] = a4[(size_t)st.corner(k1) * 10 + 5 * h1 + i];
+			}
+			const float w = st.weight(k);
+			float v[20];
+#pragma unroll
+			for (int i = 0; i < 5; ++i) {
+				const f32x4 t = cur[i];
+				v[4 * i] = t.x;
+				v[4 * i + 1] = t.y;
+				v[4 * i + 2] = t.z;
+				v[4 * i + 3] = t.w;
+			}
+#pragma unroll
+			for (int m2 = 0; m2 < 2; ++m2) {
+				float t = 0.f;
+#pragma unroll
+				for (int c = 0; c < 10; ++c)
+					t += v[10 * m2 + c] * P[c];
+				out[2 * hf + m2] += w * t;
+			}
+#pragma unroll
+			for (int i = 0; i < 5; ++i)
+				cur[i] = nxt[i];
+		}
+	}
+}
+
 /* sum_k w_k (g_{t nu}(corner k) u^nu) and sum_k w_k rho(corner k): k_t and
  * n_e of one level (interpolation commutes with the contraction). */
 __device__ __forceinline__ void load4(gptr<float> a, float v[4])
