@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "fpdiv.h"
 #include "grayhip_params.h"
@@ -166,7 +167,7 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 		T aa = a * a;
 		T zz = q.z * q.z;
 		T kk = T(0.5) * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
-		T dd, rr, r, ir, half_rdd, two_rq;
+		T dd, rr, r, ir, half_rdd, two_rq, iss;
 		if constexpr (sizeof(T) == 4) {
 			/* FP32 form: the same quantities from two reciprocal square
 			 * roots and one reciprocal (transcendentals issue at a quarter
@@ -180,6 +181,33 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 			r = rr * ir;
 			half_rdd = (T(0.5) * ir) * idd;
 			two_rq = M::div(T(2.0) * rr, fmaT(aa, zz, rr * rr));
+			iss = M::div(T(1.0), rr + aa);
+		} else if constexpr (std::is_same<M, Mth<double>>::value) {
+			/* The correctly rounded operations as sqrt_core / div_core,
+			 * which give the same bits where every operand lies in the
+			 * ranges checked below (d2 and rr in [2^-600, 2^600] put r,
+			 * dd, r dd and rr + aa z^2/rr in [2^-600, 2^701]; a^2 z^2 = 0
+			 * or in [2^-400, 2^100] keeps a^2 z^2 / rr normal); any other
+			 * lane evaluates them again with the full sequences. */
+			const T d2 = fmaT(aa, zz, kk * kk);
+			const T azz = aa * zz;
+			dd = sqrt_core(d2);
+			rr = dd + kk;
+			r  = sqrt_core(rr);
+			ir = div_core(T(1.0), r);
+			half_rdd = div_core(T(0.5), r * dd);
+			two_rq = div_core(T(2.0), rr + div_core(azz, rr));
+			iss = div_core(T(1.0), rr + aa);
+			if (!(in_range(d2, 0x1p-600, 0x1p600) && in_range(rr, 0x1p-600, 0x1p600) &&
+			      (azz == 0.0 || in_range(azz, 0x1p-400, 0x1p100)) && aa <= 0x1p100)) {
+				dd = M::sqrt_(d2);
+				rr = dd + kk;
+				r  = M::sqrt_(rr);
+				ir = M::div(T(1.0), r);
+				half_rdd = M::div(T(0.5), r * dd);
+				two_rq = M::div(T(2.0), rr + M::div(azz, rr));
+				iss = M::div(T(1.0), rr + aa);
+			}
 		} else {
 			dd = M::sqrt_(fmaT(aa, zz, kk * kk));
 			rr = dd + kk;
@@ -187,13 +215,13 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 			ir = M::div(T(1.0), r);
 			half_rdd = M::div(T(0.5), r * dd);
 			two_rq = M::div(T(2.0), rr + M::div(aa * zz, rr));
+			iss = M::div(T(1.0), rr + aa);
 		}
 		if (aux) {
 			aux->r = r;
 			aux->ir = ir;
 		}
 		T ss = rr + aa;
-		T iss = M::div(T(1.0), ss);
 		T dx_r, dy_r, dz_r, tmp2;
 
 		tmp  = half_rdd;
@@ -284,7 +312,20 @@ template <class T, class P>
 __device__ __forceinline__ T getdt(const P &p, const V4<T> &q,
                                    const V4<T> &u, T dt)
 {
-	T r   = Mth<T>::sqrt_(getrr(T(p.a), q));
+	T r;
+	if constexpr (sizeof(T) == 8) {
+		/* sqrt_core gives the correctly rounded root for d2 and rr in
+		 * [2^-600, 2^600]; other lanes take the full sequence. */
+		const T a = p.a, aa = a * a, zz = q.z * q.z;
+		const T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+		const T d2 = kk * kk + aa * zz;
+		const T rr = sqrt_core(d2) + kk;
+		r = sqrt_core(rr);
+		if (!(in_range(d2, 0x1p-600, 0x1p600) && in_range(rr, 0x1p-600, 0x1p600)))
+			r = Mth<T>::sqrt_(getrr(a, q));
+	} else {
+		r = Mth<T>::sqrt_(getrr(T(p.a), q));
+	}
 	T eps = r - T(p.r_h);
 
 	if (eps < T(p.eps_stop))
