@@ -105,6 +105,39 @@ def test_restart_keeps_shared_depths_and_rejects_others():
     assert (l1 <= 1e-4).all(), l1
 
 
+def test_fp64_lanes_outside_the_division_range():
+    """The plain FP64 form's RHS and step rule take their correctly rounded
+    divisions and square roots without the range steps only where a per-lane
+    test puts every operand in range; lanes with a^2 z^2 subnormal or below
+    2^-400 (z = 1e-160, 1e-100 near the equatorial plane), the discriminant
+    (r^2 - a^2)^2 / 4 + a^2 z^2 beyond 2^600 (r = 1e60), and z = 0 (a^2 z^2 =
+    0, in range) share waves with ordinary rays, and every lane stays bitwise
+    equal to the oracle."""
+    a, w = 0.9375, 8
+    p = O.make_params(a)
+    data, info = O.icond_all(p, w, w, 24.0, 24.0, 1000.0, 60.0, 0.0)
+    for k, z in ((0, 1e-160), (9, 1e-100)):  # a^2 z^2 subnormal / below 2^-400
+        data[k, 3], data[k, 7] = z, 0.0     # u^z = 0: z stays that small
+    data[18, 3], data[18, 7] = 0.0, 0.0     # exactly in the plane
+    for k in (27, 36):                      # r = 1e60, moving outward (dt < 0: inward)
+        e = data[k, 1:4] / np.linalg.norm(data[k, 1:4])
+        data[k, 1:4] = 1e60 * e
+        data[k, 5:8] = np.linalg.norm(data[k, 5:8]) * e
+    steps = np.zeros(w * w, dtype=np.int64)
+    with G.Render(w, w, a, precision=G.FP64) as r:
+        r.write(data)
+        for _ in range(3):
+            O.evolve_all(p, data, info, -32.0, 1024, steps)
+        r.render(-32.0, 1024, 3)
+        d1, i1 = r.read_f64()
+        s1 = r.steps()
+    assert (steps[[0, 9, 18, 27, 36]] > 0).all()
+    assert 0 < abs(d1[0, 3]) < 1e-155 and 0 < abs(d1[9, 3]) < 1e-95 and d1[18, 3] == 0.0
+    assert np.abs(d1[27, 1:4]).max() > 1e59
+    assert np.array_equal(steps, s1), f"{(steps != s1).sum()} rays differ in step count"
+    assert np.array_equal(d1, data) and np.array_equal(i1, info)
+
+
 def test_large_image_8192_squared():
     """67,108,864 rays: the SoA state (8 reals a ray) spans 4.3 GB, so
     device offsets pass 2^31 bytes.  Two intervals of 16 sub-steps (the
