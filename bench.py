@@ -170,7 +170,7 @@ def headline(G, args, out):
         "kernel_ms_is": "span from the first pool pass's start to the last pass's end "
                         "(the passes overlap on two streams); per-dispatch durations are in "
                         "the rocprofv3 kernel trace under profiles/",
-        **probe_fracs(achieved, "fp32_fma_tflops", "fp32_fma_tflops_3_waves"),
+        **probe_fracs(achieved, "fp32_fma_tflops", "fp32_fma_tflops_4_waves"),
     }
     return img32, info32, value
 

@@ -5,9 +5,9 @@
  *
  * Eight independent FMA chains per lane, 256-lane blocks (one wave per SIMD
  * each), 16 waves per SIMD, and again at the ray-pool kernels' own
- * occupancy (3 waves per SIMD for the FP32 pool, 2 for the plain FP64
- * kernel): at few waves a SIMD cannot hide an FMA's latency even with eight
- * independent chains.  One untimed launch, then one launch timed with HIP
+ * occupancy (4 waves per SIMD for the FP32 pool, 3 for the serving pool, 2
+ * for the plain FP64 kernel): at few waves a SIMD cannot hide an FMA's
+ * latency even with eight independent chains.  One untimed launch, then one launch timed with HIP
  * events.  Prints one JSON line: TFLOP/s (FMA = 2 FLOP).
  *
  * FP32: every operand a vector register (the multiplier and addend differ
@@ -92,13 +92,15 @@ int main()
 	const double f32 = rate<float, false>(16 * cus, 200000);
 	const double f32s = rate<float, true>(16 * cus, 200000);
 	const double f32_3 = rate<float, false>(3 * cus, 200000);
+	const double f32_4 = rate<float, false>(4 * cus, 200000);
 	const double f64 = rate<double, true>(16 * cus, 100000);
 	const double f64v = rate<double, false>(16 * cus, 100000);
 	const double f64_2 = rate<double, true>(2 * cus, 100000);
 	printf("{\"fp32_fma_tflops\": %.3f, \"fp32_sgpr_fma_tflops\": %.3f, "
-	       "\"fp32_fma_tflops_3_waves\": %.3f, \"fp64_fma_tflops\": %.3f, "
+	       "\"fp32_fma_tflops_3_waves\": %.3f, \"fp32_fma_tflops_4_waves\": %.3f, "
+	       "\"fp64_fma_tflops\": %.3f, "
 	       "\"fp64_vgpr_fma_tflops\": %.3f, \"fp64_fma_tflops_2_waves\": %.3f, \"cus\": %d, "
 	       "\"waves_per_simd\": 16, \"chains_per_lane\": 8}\n",
-	       f32, f32s, f32_3, f64, f64v, f64_2, cus);
+	       f32, f32s, f32_3, f32_4, f64, f64v, f64_2, cus);
 	return 0;
 }

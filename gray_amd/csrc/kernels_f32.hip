@@ -25,6 +25,38 @@
 
 namespace gray {
 
+/* The pool's render-uniform scalars, converted on the host: each stays in
+ * a scalar register, where a conversion in the kernel would hold its result
+ * in a vector register for the whole kernel.  Field names are those of
+ * RayParams that getdt and rhs_f32 read. */
+template <int NF> struct PoolConst {
+	float a, r_h, eps_stop, r_esc, r_isco;   /* (float) of RayParams' */
+	int escape, fluid;
+	float dts;                                 /* (float) sr.dts         */
+	double n_over_dt, dt_over_n;               /* sr.n / sr.dt, sr.dt / sr.n */
+	RtConst<NF> rc;
+};
+
+template <int NF>
+PoolConst<NF> make_pool_const(const RayParams &p, const StepRule &sr)
+{
+	PoolConst<NF> c;
+	c.a = (float)p.a;
+	c.r_h = (float)p.r_h;
+	c.eps_stop = (float)p.eps_stop;
+	c.r_esc = (float)p.r_esc;
+	c.r_isco = (float)p.r_isco;
+	c.escape = p.escape;
+	c.fluid = p.fluid;
+	c.dts = (float)sr.dts;
+	/* the quantised step count is gdt n / dt; the FP32 form takes it as
+	 * gdt (n / dt), one multiply instead of a division per step */
+	c.n_over_dt = (double)sr.n / sr.dt;
+	c.dt_over_n = sr.dt / (double)sr.n;
+	c.rc = make_rtconst<NF>(p);
+	return c;
+}
+
 template <int NF>
 __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
                                                   double *__restrict__ data,
@@ -32,22 +64,17 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
                                                   int64_t *__restrict__ steps,
                                                   size_t n_rays, RayRange rr,
                                                   Counters *__restrict__ ctr,
-                                                  Fallback *__restrict__ fb)
+                                                  Fallback *__restrict__ fb,
+                                                  const PoolConst<NF> pc)
 {
 	const unsigned n0 = (unsigned)(rr.e0 - rr.b0), n_all = n0 + (unsigned)(rr.e1 - rr.b1);
 	constexpr int ND = nd32<NF>();
 	const unsigned lane = __lane_id();
-	const float dts = (float)sr.dts;
-	const double dt_over_n = sr.dt / (double)sr.n;
-	/* the quantised step count is gdt n / dt; the FP32 form takes it as
-	 * gdt (n / dt), one multiply instead of a division per step */
-	const double n_over_dt = (double)sr.n / sr.dt;
-	const RtConst<NF> rc = make_rtconst<NF>(p);
 
 	float hi[ND], lo[ND];
-	long g = -1;
+	int g = -1;
 	int d = 0, h = 0;
-	int64_t st = 0, st0 = 0;
+	int dst = 0;                 /* steps taken since the ray was loaded */
 	bool exhausted = false;
 
 	for (;;) {
@@ -63,10 +90,9 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			if (need) {
 				unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
 				if (idx < n_all) {
-					g = idx < n0 ? (long)rr.b0 + idx : (long)rr.b1 + (idx - n0);
-					load_state<NF>(p, data, n_rays, g, hi, lo);
-					st = steps[g];
-					st0 = st;
+					g = idx < n0 ? rr.b0 + (int)idx : rr.b1 + (int)(idx - n0);
+					load_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
+					dst = 0;
 					d = 0;
 					h = 0;
 				} else {
@@ -79,26 +105,26 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
-		float gdt = getdt(p, q, u, dts);
-		double xd = (double)gdt * n_over_dt;
+		float gdt = getdt(pc, q, u, pc.dts);
+		double xd = (double)gdt * pc.n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		bool finish = (dh == 0);
 		if (!finish) {
 			if (dh > sr.n - h)
 				dh = sr.n - h;
-			double hd = (double)dh * dt_over_n;
+			double hd = (double)dh * pc.dt_over_n;
 			if (fabs(hd) < 1e-4) {
 				/* step collapse: FP64 re-integration from here */
-				store_state<NF>(p, data, n_rays, g, hi, lo);
-				steps[g] = st;
-				atomicAdd(&ctr->pool_steps, (unsigned long long)(st - st0));
+				store_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
+				steps[g] += dst;
+				atomicAdd(&ctr->pool_steps, (unsigned long long)dst);
 				unsigned slot = atomicAdd(&ctr->n_fb, 1u);
 				fb[slot] = Fallback{(int)g, d, h, 0};
 				g = -1;
 				continue;
 			}
-			rk4_f32c<NF>(p, rc, hi, lo, (float)hd);
-			++st;
+			rk4_f32c<NF>(pc, pc.rc, hi, lo, (float)hd);
+			++dst;
 			float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 			if (!__builtin_isfinite(chk)) {
 				/* the stored state is still the one this pass started
@@ -115,14 +141,14 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			}
 		}
 		if (finish) {
-			store_state<NF>(p, data, n_rays, g, hi, lo);
+			store_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
 			V4<double> qd = {(double)hi[0] + (double)lo[0], (double)hi[1] + (double)lo[1],
 			                 (double)hi[2] + (double)lo[2], (double)hi[3] + (double)lo[3]};
 			V4<double> ud = {(double)hi[4] + (double)lo[4], (double)hi[5] + (double)lo[5],
 			                 (double)hi[6] + (double)lo[6], (double)hi[7] + (double)lo[7]};
 			info[g] = getuu(p.a, qd, ud);
-			steps[g] = st;
-			atomicAdd(&ctr->pool_steps, (unsigned long long)(st - st0));
+			steps[g] += dst;
+			atomicAdd(&ctr->pool_steps, (unsigned long long)dst);
 			g = -1;
 		}
 	}
@@ -161,7 +187,7 @@ hipError_t launch_pool_f32(const RayParams &p, const StepRule &sr, double *data,
 	if (rr.e0 - rr.b0 + rr.e1 - rr.b1 <= 0 || n_blocks == 0)
 		return hipSuccess;
 #define L(NF) hipLaunchKernelGGL(k_pool_f32<NF>, dim3(n_blocks), dim3(256), 0, st, \
-	                         p, sr, data, info, steps, n_rays, rr, ctr, fb)
+	                         p, sr, data, info, steps, n_rays, rr, ctr, fb, make_pool_const<NF>(p, sr))
 	GRAY_NF_SWITCH(p.n_freq, L)
 #undef L
 	return hipGetLastError();

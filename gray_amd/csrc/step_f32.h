@@ -28,7 +28,7 @@ template <int NF> struct RtConst {
 };
 
 template <int NF, class P>
-__device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
+__host__ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
 {
 	RtConst<NF> r;
 #pragma unroll
