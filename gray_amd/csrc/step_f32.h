@@ -20,11 +20,12 @@ namespace gray {
 /* Reals of the FP32 form's state: geodesic, intensities, T. */
 template <int NF> constexpr int nd32() { return 8 + NF + (NF > 0 ? 1 : 0); }
 
-/* Per-channel constants of the synthetic emissivity, nu_i^-1/2 and
- * 0.1 nu_i^-5/2, so that a transfer RHS needs one rsqrt for the redshift
- * and one exp per channel. */
+/* Per-channel constants of the synthetic emissivity in base 2:
+ * l2c_i = log2 nu_i^-1/2 and d2_i = 0.1 nu_i^-5/2 log2(e), so that a
+ * channel's factor nu_i^-1/2 exp(-tau_i) = 2^(l2c_i - d2_i T) is one FMA
+ * and one exp2. */
 template <int NF> struct RtConst {
-	float c[NF > 0 ? NF : 1], d[NF > 0 ? NF : 1];
+	float l2c[NF > 0 ? NF : 1], d2[NF > 0 ? NF : 1];
 };
 
 template <int NF, class P>
@@ -34,8 +35,8 @@ __host__ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
 #pragma unroll
 	for (int i = 0; i < (NF > 0 ? NF : 1); ++i) {
 		const double nu = NF > 0 ? (double)p.nu[i] : 1.0;
-		r.c[i] = (float)(1.0 / sqrt(nu));
-		r.d[i] = (float)(0.1 / (nu * nu * sqrt(nu)));
+		r.l2c[i] = (float)(-0.5 * log2(nu));
+		r.d2[i] = (float)(0.1 / (nu * nu * sqrt(nu)) * 1.4426950408889634);
 	}
 	return r;
 }
@@ -46,7 +47,8 @@ __host__ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
  * for the Keplerian v = (1, -y Omega, x Omega, 0); then
  *   dI_i = -n nu_i^-1/2 s^-1/2 exp(-tau_i) / (s^2 + 1e-28)
  *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2       (s = -k.u_fluid),
- * with tau_i = d_i T and dT = -n s^-3/2. */
+ * with tau_i = d_i T and dT = -n s^-3/2; the exps run as exp2 of base-2
+ * arguments (v_exp_f32, as __expf). */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, const V4<float> &q,
                                            const V4<float> &u, const KsAux<float> &K, float T,
@@ -71,7 +73,10 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 				const float s_ = -kv * ut;
 				if (s_ > 0.f) {
 					sh = s_;
-					n = (K.ir * isr) * M::exp_(M::div(-(q.z * q.z), 0.18f * R2));
+					/* exp(-z^2 / (0.18 R^2)) as exp2 */
+					n = (K.ir * isr) *
+					    __builtin_amdgcn_exp2f((q.z * q.z) * __builtin_amdgcn_rcpf(R2) *
+					                           (-1.4426950408889634f / 0.18f));
 				}
 			}
 		}
@@ -90,7 +95,7 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 	dT = -(n * (rsh * rsh) * rsh);
 #pragma unroll
 	for (int i = 0; i < NF; ++i)
-		dI[i] = -(A * rc.c[i]) * M::exp_(-(rc.d[i] * T));
+		dI[i] = -A * __builtin_amdgcn_exp2f(fmaf(-rc.d2[i], T, rc.l2c[i]));
 }
 
 template <int NF, class P>
