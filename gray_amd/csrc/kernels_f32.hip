@@ -70,12 +70,17 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 	const unsigned n0 = (unsigned)(rr.e0 - rr.b0), n_all = n0 + (unsigned)(rr.e1 - rr.b1);
 	constexpr int ND = nd32<NF>();
 	const unsigned lane = __lane_id();
-	/* the channels' l2c in vector registers for the whole kernel: an FMA
-	 * takes one scalar operand, and d2 is the other */
+	/* the channels' l2c in vector registers for the whole kernel (an FMA
+	 * takes one scalar operand, and d2 is the other), and the spin, which
+	 * the RHS reads 18 times per evaluation (an FP32 instruction reading a
+	 * scalar register issues slower); the register moves are opaque to
+	 * the compiler, which would otherwise repeat them inside the loop */
 	RtConst<NF> rc = pc.rc;
 #pragma unroll
 	for (int i = 0; i < NF; ++i)
 		asm("v_mov_b32 %0, %1" : "=v"(rc.l2c[i]) : "s"(pc.rc.l2c[i]));
+	PoolConst<NF> pv = pc;
+	asm("v_mov_b32 %0, %1" : "=v"(pv.a) : "s"(pc.a));
 
 	float hi[ND], lo[ND];
 	int g = -1;
@@ -111,7 +116,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
-		float gdt = getdt(pc, q, u, pc.dts);
+		float gdt = getdt(pv, q, u, pc.dts);
 		double xd = (double)gdt * pc.n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		bool finish = (dh == 0);
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 				g = -1;
 				continue;
 			}
-			rk4_f32c<NF>(pc, rc, hi, lo, (float)hd);
+			rk4_f32c<NF>(pv, rc, hi, lo, (float)hd);
 			++dst;
 			float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 			if (!__builtin_isfinite(chk)) {
