@@ -104,6 +104,29 @@ __device__ __forceinline__ T getuu(T a, const V4<T> &q, const V4<T> &u)
 	return dot4(down(a, q, u), u);
 }
 
+/* The inverse of down(): u^mu = g^{mu nu} p_nu with g^{mu nu} = eta^{mu nu}
+ * - f l^mu l^nu, l^mu = (-1, lx, ly, lz) (l is null), so u^mu = eta^{mu nu}
+ * p_nu - f (l^nu p_nu) l^mu; f and l as in down(). */
+template <class T>
+__device__ __forceinline__ V4<T> up(T a, const V4<T> &q, const V4<T> &p)
+{
+	typedef Mth<T> M;
+	T aa = a * a;
+	T zz = q.z * q.z;
+	T kk = T(0.5) * (((q.x * q.x + q.y * q.y) + zz) - aa);
+	T rr = M::sqrt_(kk * kk + aa * zz) + kk;
+	T r  = M::sqrt_(rr);
+
+	T f  = M::div((T(2.0) * rr) * r, rr * rr + aa * zz);
+	T lx = M::div(r * q.x + a * q.y, rr + aa);
+	T ly = M::div(r * q.y - a * q.x, rr + aa);
+	T lz = M::div(q.z, r);
+
+	T fL = f * (((lx * p.x + ly * p.y) + lz * p.z) - p.t);
+	V4<T> u = {fL - p.t, p.x - fL * lx, p.y - fL * ly, p.z - fL * lz};
+	return u;
+}
+
 /* sim-org/KS.cl:188-232 with the camera trigonometry passed in. */
 template <class T>
 __device__ __forceinline__ void gr_icond(T a, T r_obs, T si, T ci, T sj, T cj,
@@ -151,13 +174,15 @@ template <class T> struct KsAux {
 	T r, ir, f, lx, ly, lz, lu;   /* ir = 1 / r */
 };
 
-/* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS.
- * Returns du/dlambda (dq/dlambda = u); fills *aux when given.  M supplies
- * sqrt and division (the correctly rounded ones by default). */
+/* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS in
+ * FP64 (the FP32 form integrates the Hamiltonian form below).  Returns
+ * du/dlambda (dq/dlambda = u); fills *aux when given.  M supplies sqrt and
+ * division (the correctly rounded ones by default). */
 template <class T, class M = Mth<T>>
 __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
                                         KsAux<T> *aux = nullptr)
 {
+	static_assert(sizeof(T) == 8, "gr_rhs is the FP64 RHS");
 	T f, dx_f, dy_f, dz_f;
 	T lx, dx_lx, dy_lx, dz_lx;
 	T ly, dx_ly, dy_ly, dz_ly;
@@ -168,21 +193,7 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 		T zz = q.z * q.z;
 		T kk = T(0.5) * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
 		T dd, rr, r, ir, half_rdd, two_rq, iss;
-		if constexpr (sizeof(T) == 4) {
-			/* FP32 form: the same quantities from two reciprocal square
-			 * roots and one reciprocal (transcendentals issue at a quarter
-			 * of the VALU rate) instead of two square roots and four
-			 * reciprocals; 2 / (rr + aa zz / rr) = 2 rr / (rr^2 + aa zz). */
-			const T d2 = fmaT(aa, zz, kk * kk);
-			const T idd = __builtin_amdgcn_rsqf(d2);
-			dd = d2 * idd;
-			rr = dd + kk;
-			ir = __builtin_amdgcn_rsqf(rr);
-			r = rr * ir;
-			half_rdd = (T(0.5) * ir) * idd;
-			two_rq = M::div(T(2.0) * rr, fmaT(aa, zz, rr * rr));
-			iss = M::div(T(1.0), rr + aa);
-		} else if constexpr (std::is_same<M, Mth<double>>::value) {
+		if constexpr (std::is_same<M, Mth<double>>::value) {
 			/* The correctly rounded operations as sqrt_core / div_core,
 			 * which give the same bits where every operand lies in the
 			 * ranges checked below (d2 and rr in [2^-600, 2^600] put r,
@@ -303,6 +314,116 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 	V4<T> acc = {uD.t - tmp, fmaT(lx, tmp, hDxu - uD.x),
 	             fmaT(ly, tmp, hDyu - uD.y), fmaT(lz, tmp, hDzu - uD.z)};
 	return acc;
+}
+
+/* f, l_i and their first derivatives at q in FP32: the quantities of the
+ * first half of gr_rhs, from two reciprocal square roots and two
+ * reciprocals (transcendentals issue at a quarter of the VALU rate) instead
+ * of two square roots and four reciprocals; 2 / (rr + aa zz / rr) =
+ * 2 rr / (rr^2 + aa zz).  d{j}_{g} is the j-derivative of g. */
+struct KsFieldsF32 {
+	float r, ir, f, dx_f, dy_f, dz_f;
+	float lx, dx_lx, dy_lx, dz_lx;
+	float ly, dx_ly, dy_ly, dz_ly;
+	float lz, dx_lz, dy_lz, dz_lz;
+};
+
+__device__ __forceinline__ KsFieldsF32 ks_fields_f32(float a, const V4<float> &q)
+{
+	typedef Mth<float> M;
+	KsFieldsF32 F;
+	const float aa = a * a;
+	const float zz = q.z * q.z;
+	const float kk = 0.5f * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
+	const float d2 = fmaT(aa, zz, kk * kk);
+	const float idd = __builtin_amdgcn_rsqf(d2);
+	const float dd = d2 * idd;
+	const float rr = dd + kk;
+	const float ir = __builtin_amdgcn_rsqf(rr);
+	const float r = rr * ir;
+	const float half_rdd = (0.5f * ir) * idd;
+	const float two_rq = M::div(2.0f * rr, fmaT(aa, zz, rr * rr));
+	const float ss = rr + aa;
+	const float iss = M::div(1.0f, ss);
+	F.r = r;
+	F.ir = ir;
+
+	float tmp = half_rdd;
+	const float dz_r = (tmp * ss) * q.z;
+	tmp = tmp * rr;
+	const float dy_r = tmp * q.y;
+	const float dx_r = tmp * q.x;
+	tmp = two_rq;
+	const float tmp2 = fmaT(-(2.0f * rr), tmp, 3.0f);
+	F.f = tmp * r;
+	F.dx_f = (tmp * dx_r) * tmp2;
+	F.dy_f = (tmp * dy_r) * tmp2;
+	F.dz_f = tmp * fmaT(-((tmp * aa) * q.z), ir, dz_r * tmp2);
+
+	const float m2r = -2.0f * r;
+	const float issr = iss * r;
+	const float issa = iss * a;
+
+	F.lx = iss * fmaT(q.y, a, q.x * r);
+	tmp = iss * fmaT(m2r, F.lx, q.x);
+	F.dx_lx = fmaT(tmp, dx_r, issr);
+	F.dy_lx = fmaT(tmp, dy_r, issa);
+	F.dz_lx = tmp * dz_r;
+
+	F.ly = iss * fmaT(-q.x, a, q.y * r);
+	tmp = iss * fmaT(m2r, F.ly, q.y);
+	F.dx_ly = fmaT(tmp, dx_r, -issa);
+	F.dy_ly = fmaT(tmp, dy_r, issr);
+	F.dz_ly = tmp * dz_r;
+
+	F.lz = q.z * ir;
+	tmp = (-F.lz) * ir;
+	F.dx_lz = tmp * dx_r;
+	F.dy_lz = tmp * dy_r;
+	F.dz_lz = fmaT(tmp, dz_r, ir);
+	return F;
+}
+
+/* The FP32 form's geodesic RHS: the Hamiltonian form of the same Kerr-Schild
+ * geodesics (sim-org/KS.cl:263-378 integrates u^mu), with the covariant
+ * momentum p_mu = g_{mu nu} u^nu as the state.  H = g^{mu nu} p_mu p_nu / 2,
+ * g^{mu nu} = eta^{mu nu} - f l^mu l^nu, l^mu = (-1, lx, ly, lz), L = l^mu p_mu:
+ *   dx^mu/dlambda = dH/dp_mu = eta^{mu nu} p_nu - f L l^mu,
+ *   dp_i/dlambda = -dH/dx^i = L (L d_i f / 2 + f (d_i l^nu) p_nu),
+ *   dp_t/dlambda = 0 (the metric is stationary),
+ * which needs f, l and their first derivatives but no Christoffel
+ * contraction.  Fills *aux (aux->lu = L = l_mu u^mu, as l is null). */
+struct HamF32 {
+	V4<float> u;             /* dx^mu / dlambda     */
+	float dpx, dpy, dpz;     /* dp_i / dlambda      */
+};
+
+__device__ __forceinline__ HamF32 ks_ham_f32(float a, const V4<float> &q, const V4<float> &p,
+                                             KsAux<float> *aux = nullptr)
+{
+	const KsFieldsF32 F = ks_fields_f32(a, q);
+	const float L = fmaT(F.lz, p.z, fmaT(F.ly, p.y, fmaT(F.lx, p.x, -p.t)));
+	const float fL = F.f * L;
+	HamF32 h;
+	h.u = V4<float>{fL - p.t, fmaT(-fL, F.lx, p.x), fmaT(-fL, F.ly, p.y),
+	                fmaT(-fL, F.lz, p.z)};
+	const float hL = 0.5f * L;
+	const float gx = fmaT(F.dx_lz, p.z, fmaT(F.dx_ly, p.y, F.dx_lx * p.x));
+	const float gy = fmaT(F.dy_lz, p.z, fmaT(F.dy_ly, p.y, F.dy_lx * p.x));
+	const float gz = fmaT(F.dz_lz, p.z, fmaT(F.dz_ly, p.y, F.dz_lx * p.x));
+	h.dpx = L * fmaT(hL, F.dx_f, F.f * gx);
+	h.dpy = L * fmaT(hL, F.dy_f, F.f * gy);
+	h.dpz = L * fmaT(hL, F.dz_f, F.f * gz);
+	if (aux) {
+		aux->r = F.r;
+		aux->ir = F.ir;
+		aux->f = F.f;
+		aux->lx = F.lx;
+		aux->ly = F.ly;
+		aux->lz = F.lz;
+		aux->lu = L;
+	}
+	return h;
 }
 
 /* sim-org/phys.cl:48-64 with the selectable escape rule.  P is any

@@ -114,8 +114,11 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 		if (g < 0)
 			break;
 
+		/* the first RK4 stage: its dx/dlambda is the u^mu of the step rule */
+		float k1[ND];
+		rhs_f32<NF>(pv, rc, hi, k1);
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
-		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
+		V4<float> u = {k1[0], k1[1], k1[2], k1[3]};
 		float gdt = getdt(pv, q, u, pc.dts);
 		double xd = (double)gdt * pc.n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 				g = -1;
 				continue;
 			}
-			rk4_f32c<NF>(pv, rc, hi, lo, (float)hd);
+			rk4_f32c<NF>(pv, rc, hi, lo, k1, (float)hd);
 			++dst;
 			float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 			if (!__builtin_isfinite(chk)) {
@@ -152,12 +155,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			}
 		}
 		if (finish) {
-			store_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
-			V4<double> qd = {(double)hi[0] + (double)lo[0], (double)hi[1] + (double)lo[1],
-			                 (double)hi[2] + (double)lo[2], (double)hi[3] + (double)lo[3]};
-			V4<double> ud = {(double)hi[4] + (double)lo[4], (double)hi[5] + (double)lo[5],
-			                 (double)hi[6] + (double)lo[6], (double)hi[7] + (double)lo[7]};
-			info[g] = getuu(p.a, qd, ud);
+			info[g] = store_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
 			steps[g] += dst;
 			atomicAdd(&ctr->pool_steps, (unsigned long long)dst);
 			g = -1;

@@ -187,7 +187,7 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 				seg_item(A.seg, C.seg0, C.seg1, fidx, tab, ray);
 				double x[8 + 2 * NF];
 				icond_ray<NF, RayParams>(reqs[tab].p, (size_t)ray, x);
-				split_state<NF, SrvClass>(C, x, hi, lo);
+				split_state<NF, SrvClass>(C, reqs[tab].p.a, x, hi, lo);
 				d = 0;
 				h = 0;
 				st = 0;
@@ -218,8 +218,11 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
 			break;
 		}
+		/* the first RK4 stage: its dx/dlambda is the u^mu of the step rule */
+		float k1[ND];
+		rhs_f32<NF>(P, rc, hi, k1);
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
-		V4<float> u = {hi[4], hi[5], hi[6], hi[7]};
+		V4<float> u = {k1[0], k1[1], k1[2], k1[3]};
 		const float gdt = getdt(P, q, u, dts);
 		const double xd = (double)gdt * n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
@@ -233,7 +236,7 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			if (fabs(hd) < 1e-4) {
 				state = 2;
 			} else {
-				rk4_f32c<NF>(P, rc, hi, lo, (float)hd);
+				rk4_f32c<NF>(P, rc, hi, lo, k1, (float)hd);
 				++st;
 				const float chk = ((hi[1] + hi[2]) + (hi[3] + hi[4])) + ((hi[5] + hi[6]) + hi[7]);
 				if (!__builtin_isfinite(chk)) {
@@ -254,7 +257,7 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 		const SrvRequest &R = reqs[tab];
 		if (state == 1) {
 			double x[8 + 2 * NF];
-			join_state<NF, SrvClass>(C, hi, lo, x);
+			join_state<NF, SrvClass>(C, R.p.a, hi, lo, x);
 			V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
 			finish_ray(R, tab, ray, x, NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr, A.done);
 		} else {
@@ -266,6 +269,9 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 #pragma unroll
 			for (int i = 0; i < 8 + NF; ++i)
 				x[i] = (double)hi[i] + (double)lo[i];
+			const V4<double> ud = up(R.p.a, V4<double>{x[0], x[1], x[2], x[3]},
+			                         V4<double>{x[4], x[5], x[6], x[7]});
+			x[4] = ud.t; x[5] = ud.x; x[6] = ud.y; x[7] = ud.z;
 #pragma unroll
 			for (int i = 8 + NF; i < SRV_NC; ++i)
 				x[i] = 0.0;

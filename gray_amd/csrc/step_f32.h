@@ -7,9 +7,14 @@
  * shift = 0.1 nu_i^-5/2 (n shift^-3/2), so every channel's optical depth is
  * tau_i = d_i T with d_i = 0.1 nu_i^-5/2 and one shared integral
  * T = -int n shift^-3/2 dlambda.  The FP32 form integrates T once instead of
- * the n_freq depths: its state is 8 geodesic reals, the n_freq intensities
- * and T (ND32 reals); load_state and store_state convert from and to the
- * stored layout (8 + 2 n_freq reals, tau_i in place).
+ * the n_freq depths.
+ *
+ * Its geodesic is the Hamiltonian form (ks_ham_f32): the state holds the
+ * covariant momentum p_mu instead of u^mu, so a ray's state is t, x, y, z,
+ * p_t, p_x, p_y, p_z, the n_freq intensities and T (ND32 reals); p_t is
+ * conserved and not integrated.  load_state and store_state convert from
+ * and to the stored layout (t, x, y, z, u^mu, intensities, tau_i in place)
+ * in FP64 (down / up).
  */
 #pragma once
 
@@ -19,6 +24,9 @@ namespace gray {
 
 /* Reals of the FP32 form's state: geodesic, intensities, T. */
 template <int NF> constexpr int nd32() { return 8 + NF + (NF > 0 ? 1 : 0); }
+
+/* Index of p_t in it (constant along the ray). */
+constexpr int PT32 = 4;
 
 /* Per-channel constants of the synthetic emissivity in base 2:
  * l2c_i = log2 nu_i^-1/2 and d2_i = 0.1 nu_i^-5/2 log2(e), so that a
@@ -42,8 +50,8 @@ __host__ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
 }
 
 /* The transfer RHS of sim-org/rt.cl:185-198 with the synthetic disk,
- * evaluated from the Kerr-Schild quantities gr_rhs already has:
- *   k_mu = eta u + f (l.u) l_mu,   g(v,v) = eta(v,v) + f (l.v)^2
+ * evaluated from the state's covariant k_mu = p_mu and the Kerr-Schild
+ * quantities the geodesic RHS already has: g(v,v) = eta(v,v) + f (l.v)^2
  * for the Keplerian v = (1, -y Omega, x Omega, 0); then
  *   dI_i = -n nu_i^-1/2 s^-1/2 exp(-tau_i) / (s^2 + 1e-28)
  *   dtau_i = -0.1 n nu_i^-5/2 s^-3/2       (s = -k.u_fluid),
@@ -51,7 +59,7 @@ __host__ __device__ __forceinline__ RtConst<NF> make_rtconst(const P &p)
  * arguments (v_exp_f32, as __expf). */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, const V4<float> &q,
-                                           const V4<float> &u, const KsAux<float> &K, float T,
+                                           const V4<float> &k, const KsAux<float> &K, float T,
                                            float *dI, float &dT)
 {
 	typedef Mth<float> M;
@@ -67,9 +75,7 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 			const float vv = (Om * Om * R2 - 1.f) + K.f * lv * lv;
 			if (vv < 0.f) {
 				const float ut = __builtin_amdgcn_rsqf(-vv);
-				const float flu = K.f * K.lu;
-				const float kx = u.x + flu * K.lx, ky = u.y + flu * K.ly;
-				const float kv = (flu - u.t) + Om * (q.x * ky - q.y * kx);
+				const float kv = k.t + Om * (q.x * k.y - q.y * k.x);
 				const float s_ = -kv * ut;
 				if (s_ > 0.f) {
 					sh = s_;
@@ -103,40 +109,45 @@ __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const
                                         float *ds)
 {
 	V4<float> q = {s[0], s[1], s[2], s[3]};
-	V4<float> u = {s[4], s[5], s[6], s[7]};
+	V4<float> k = {s[4], s[5], s[6], s[7]};
 	KsAux<float> K;
-	V4<float> a = gr_rhs((float)p.a, q, u, NF > 0 ? &K : nullptr);
-	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
-	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
+	const HamF32 h = ks_ham_f32((float)p.a, q, k, NF > 0 ? &K : nullptr);
+	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
+	ds[4] = 0.f;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
 	if constexpr (NF > 0)
-		rt_rhs_f32<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
+		rt_rhs_f32<NF, P>(p, rc, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
 /* Classical RK4 (sim-org/RK4.cl:37-47) with stages taken from hi and the
- * increment added to (hi, lo) by compensated summation. */
+ * increment added to (hi, lo) by compensated summation.  k1 is the first
+ * stage, rhs_f32 at hi, which the caller has evaluated (its dx/dlambda gives
+ * the step rule u^mu); p_t (PT32) is conserved and left as it is. */
 template <int NF, class P>
 __device__ __forceinline__ void rk4_f32c(const P &p, const RtConst<NF> &rc, float *hi, float *lo,
-                                         float dt)
+                                         const float *k1, float dt)
 {
 	constexpr int ND = nd32<NF>();
 	float k[ND], acc[ND], x[ND];
 	const float hdt = 0.5f * dt;
 
-	rhs_f32<NF, P>(p, rc, hi, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
-		acc[i] = k[i];
-		x[i] = hi[i] + hdt * k[i];
+		acc[i] = k1[i];
+		x[i] = i == PT32 ? hi[i] : hi[i] + hdt * k1[i];
 	}
 	rhs_f32<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
+		if (i == PT32)
+			continue;
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + hdt * k[i];
 	}
 	rhs_f32<NF, P>(p, rc, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
+		if (i == PT32)
+			continue;
 		acc[i] += 2.0f * k[i];
 		x[i] = hi[i] + dt * k[i];
 	}
@@ -144,6 +155,8 @@ __device__ __forceinline__ void rk4_f32c(const P &p, const RtConst<NF> &rc, floa
 	const float dt6 = dt * (1.0f / 6.0f);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
+		if (i == PT32)
+			continue;
 		float y = (acc[i] + k[i]) * dt6 + lo[i];
 		float t = hi[i] + y;
 		lo[i] = y - (t - hi[i]);
@@ -161,29 +174,40 @@ __device__ __forceinline__ double tau_scale(const P &p, int i)
 }
 
 /* Stored layout (x: 8 + 2 NF reals) -> FP32-form state (hi + lo, ND32
- * reals): T from channel 0's depth. */
+ * reals): p_mu = down(u) at spin a, T from channel 0's depth. */
 template <int NF, class P>
-__device__ __forceinline__ void split_state(const P &p, const double *x, float *hi, float *lo)
+__device__ __forceinline__ void split_state(const P &p, double a, const double *x, float *hi,
+                                            float *lo)
 {
+	const V4<double> q = {x[0], x[1], x[2], x[3]};
+	const V4<double> k = down(a, q, V4<double>{x[4], x[5], x[6], x[7]});
+	double y[8 + NF + 1];
+	y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; y[3] = x[3];
+	y[4] = k.t;  y[5] = k.x;  y[6] = k.y;  y[7] = k.z;
 #pragma unroll
-	for (int i = 0; i < 8 + NF; ++i) {
-		hi[i] = (float)x[i];
-		lo[i] = (float)(x[i] - (double)hi[i]);
-	}
-	if constexpr (NF > 0) {
-		const double T = x[8 + NF] / tau_scale(p, 0);
-		hi[8 + NF] = (float)T;
-		lo[8 + NF] = (float)(T - (double)hi[8 + NF]);
+	for (int i = 8; i < 8 + NF; ++i)
+		y[i] = x[i];
+	if constexpr (NF > 0)
+		y[8 + NF] = x[8 + NF] / tau_scale(p, 0);
+#pragma unroll
+	for (int i = 0; i < nd32<NF>(); ++i) {
+		hi[i] = (float)y[i];
+		lo[i] = (float)(y[i] - (double)hi[i]);
 	}
 }
 
-/* FP32-form state -> stored layout (8 + 2 NF reals, tau_i = d_i T). */
+/* FP32-form state -> stored layout (8 + 2 NF reals: u^mu = up(p) at spin
+ * a, tau_i = d_i T). */
 template <int NF, class P>
-__device__ __forceinline__ void join_state(const P &p, const float *hi, const float *lo, double *x)
+__device__ __forceinline__ void join_state(const P &p, double a, const float *hi, const float *lo,
+                                           double *x)
 {
 #pragma unroll
 	for (int i = 0; i < 8 + NF; ++i)
 		x[i] = (double)hi[i] + (double)lo[i];
+	const V4<double> u = up(a, V4<double>{x[0], x[1], x[2], x[3]},
+	                        V4<double>{x[4], x[5], x[6], x[7]});
+	x[4] = u.t; x[5] = u.x; x[6] = u.y; x[7] = u.z;
 	if constexpr (NF > 0) {
 		const double T = (double)hi[8 + NF] + (double)lo[8 + NF];
 #pragma unroll
@@ -200,18 +224,21 @@ __device__ __forceinline__ void load_state(const P &p, const double *__restrict_
 #pragma unroll
 	for (int i = 0; i < 8 + NF + (NF > 0 ? 1 : 0); ++i)
 		x[i] = data[(size_t)i * n_rays + g];
-	split_state<NF, P>(p, x, hi, lo);
+	split_state<NF, P>(p, p.a, x, hi, lo);
 }
 
+/* Writes the stored layout and returns g(u,u) of the stored state. */
 template <int NF, class P>
-__device__ __forceinline__ void store_state(const P &p, double *__restrict__ data, size_t n_rays,
-                                            size_t g, const float *hi, const float *lo)
+__device__ __forceinline__ double store_state(const P &p, double *__restrict__ data,
+                                              size_t n_rays, size_t g, const float *hi,
+                                              const float *lo)
 {
 	double x[8 + 2 * NF];
-	join_state<NF, P>(p, hi, lo, x);
+	join_state<NF, P>(p, p.a, hi, lo, x);
 #pragma unroll
 	for (int i = 0; i < 8 + 2 * NF; ++i)
 		data[(size_t)i * n_rays + g] = x[i];
+	return getuu(p.a, V4<double>{x[0], x[1], x[2], x[3]}, V4<double>{x[4], x[5], x[6], x[7]});
 }
 
 } // namespace gray
