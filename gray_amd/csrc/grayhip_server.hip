@@ -20,6 +20,7 @@
 #include <new>
 
 #include "launch_serve.h"
+#include "step_f32.h"
 #include "spacetime.h"
 #include "srv_sched.h"
 
@@ -345,6 +346,18 @@ public:
 			C.sr.n_dump = (int)cp.key.n_dump;
 			for (int i = 0; i < SRV_NF; ++i)
 				C.nu[i] = cp.key.nu[i];
+			/* the FP32 step's class constants, converted here so that the
+			 * kernel holds them in scalar registers */
+			C.n_over_dt = (double)C.sr.n / C.sr.dt;
+			C.dt_over_n = C.sr.dt / (double)C.sr.n;
+			C.dts = (float)C.sr.dts;
+			{
+				const RtConst<SRV_NF> rc = make_rtconst<SRV_NF>(C);
+				for (int i = 0; i < SRV_NF; ++i) {
+					C.l2c[i] = rc.l2c[i];
+					C.d2[i] = rc.d2[i];
+				}
+			}
 			C.r_esc = cp.key.r_esc;
 			C.n_freq = cp.key.n_freq;
 			C.fluid = cp.key.n_freq > 0 ? cp.key.fluid : GRAYHIP_FLUID_NONE;

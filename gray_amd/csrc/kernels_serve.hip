@@ -137,11 +137,19 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
                                            const SrvClass &C, int c)
 {
 	constexpr int ND = nd32<NF>();
-	const RtConst<NF> rc = make_rtconst<NF>(C);
+	/* the class's constants, converted on the host (scalar registers);
+	 * l2c in vector registers, since the channel FMA's other constant
+	 * operand is d2 */
+	RtConst<NF> rc;
+#pragma unroll
+	for (int i = 0; i < NF; ++i) {
+		asm("v_mov_b32 %0, %1" : "=v"(rc.l2c[i]) : "s"(C.l2c[i]));
+		rc.d2[i] = C.d2[i];
+	}
 	const StepRule sr = C.sr;
-	const float dts = (float)sr.dts;
-	const double dt_over_n = sr.dt / (double)sr.n;
-	const double n_over_dt = (double)sr.n / sr.dt;   /* gdt n / dt as one multiply */
+	const float dts = C.dts;
+	const double dt_over_n = C.dt_over_n;
+	const double n_over_dt = C.n_over_dt;   /* gdt n / dt as one multiply */
 	const unsigned n_resume = (unsigned)C.n_resume, n_fresh = C.n_fresh;
 	LaneKerr P;
 	P.r_esc = (float)C.r_esc;

@@ -54,6 +54,12 @@ struct SrvRequest {
 struct SrvClass {
 	StepRule sr;
 	double nu[SRV_NF];    /* channel frequencies, channel 0 repeated         */
+	double n_over_dt;     /* sr.n / sr.dt, sr.dt / sr.n                      */
+	double dt_over_n;
+	float l2c[SRV_NF];    /* the FP32 transfer's constants (make_rtconst)    */
+	float d2[SRV_NF];
+	float dts;            /* (float) sr.dts                                  */
+	float pad_f;
 	double r_esc;
 	int n_freq, fluid, escape, pad;
 	int block0, block1;   /* blocks [block0, block1) serve this class        */
