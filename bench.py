@@ -55,6 +55,16 @@ def f_rt(n_freq):
 
 
 FLOP_PER_STEP = F_KS + f_rt(len(NUS))
+# Hardware FLOP per ray-step of the kernels themselves on the headline
+# workload: rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 (FP64) x 64 lanes / ray-steps
+# (FMA = 2; profiles/kerr_disk_1024x1024_kernels.md).  The FP32 form
+# integrates the Hamiltonian form of the geodesic, which needs fewer
+# operations per ray-step than the reference's u^mu form that FLOP_PER_STEP
+# counts; the plain FP64 form evaluates the reference's form, plus the
+# Newton steps of its correctly rounded divisions and square roots.
+HW_FLOP_POOL = 890
+HW_FLOP_FP64 = 2412
+HW_SOURCE = "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32/FP64 x 64 / ray-steps (profiles/)"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBPS = 8000.0
@@ -64,7 +74,7 @@ PEAK_HBM_GBPS = 8000.0
 # profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
 # profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
 # k_grid_tiles_f64<0, double>).
-TRAFFIC_POOL_PASS = (58.24 + 80.18) * 2**20
+TRAFFIC_POOL_PASS = (73.05 + 85.62) * 2**20
 TRAFFIC_GRID_TILES = (526.93 + 21.78) * 2**20
 TRAFFIC_GRID_TILES64 = (4695.65 + 22.24) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"
@@ -100,6 +110,13 @@ def fma_probe():
         except (OSError, subprocess.SubprocessError, ValueError, IndexError) as e:
             _FMA_PROBE = {"error": repr(e)}
     return _FMA_PROBE
+
+
+def hw_roofline(achieved, flop, peak, key, key_occ):
+    """The kernel's own arithmetic rate (hardware FLOP counted by rocprofv3)
+    against the peak and against the FMA probe."""
+    return {"flop_per_ray_step": flop, "source": HW_SOURCE, "achieved": achieved,
+            "unit": "TFLOP/s", "frac": achieved / peak, **probe_fracs(achieved, key, key_occ)}
 
 
 def probe_fracs(achieved, key, key_occ):
@@ -166,11 +183,13 @@ def headline(G, args, out):
         "traffic": TRAFFIC_POOL_PASS, "traffic_unit": "B per pass",
         "traffic_source": TRAFFIC_SOURCE,
         "flop_per_ray_step": FLOP_PER_STEP,
+        "flop_per_ray_step_is": "the reference's u^mu-form operation count per ray-step",
         "kernel_ms": 1e3 * k_s,
         "kernel_ms_is": "span from the first pool pass's start to the last pass's end "
                         "(the passes overlap on two streams); per-dispatch durations are in "
                         "the rocprofv3 kernel trace under profiles/",
-        **probe_fracs(achieved, "fp32_fma_tflops", "fp32_fma_tflops_4_waves"),
+        "hw": hw_roofline((pool_steps / args.steps) * HW_FLOP_POOL / k_s / 1e12, HW_FLOP_POOL,
+                          PEAK_FP32_TFLOPS, "fp32_fma_tflops", "fp32_fma_tflops_4_waves"),
     }
     return img32, info32, value
 
@@ -190,7 +209,9 @@ def fp64_section(G, img32, info32, value, out):
         "roofline": {"bound": "valu", "kernel": "k_evolve_f64<4>", "achieved": a64,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS,
                      "kernel_ms": kn64 * 1e-6, "kernel_ms_is": "sum of the evolve launches",
-                     **probe_fracs(a64, "fp64_fma_tflops", "fp64_fma_tflops_2_waves")},
+                     "hw": hw_roofline(st64 * HW_FLOP_FP64 / (kn64 * 1e-9) / 1e12, HW_FLOP_FP64,
+                                       PEAK_FP64_TFLOPS, "fp64_fma_tflops",
+                                       "fp64_fma_tflops_2_waves")},
     }
     out["speedup_fp32_vs_fp64"] = value / fp64_rate
     out["image_rel_l1_vs_fp64"] = l1(img32, img64)
@@ -509,7 +530,10 @@ def precision_section(G, out):
         "fallback_rays": nfb,
         "roofline_fp32_pool": {"kernel": "k_pool_f32<4>", "achieved": a32, "peak": PEAK_FP32_TFLOPS,
                                "unit": "TFLOP/s", "frac": a32 / PEAK_FP32_TFLOPS,
-                               "kernel_ms": 1e-6 * kn, "kernel_ms_is": "span of the pool passes"},
+                               "kernel_ms": 1e-6 * kn, "kernel_ms_is": "span of the pool passes",
+                               "hw_frac": ps * HW_FLOP_POOL / (kn * 1e-9) / 1e12 / PEAK_FP32_TFLOPS,
+                               "hw_frac_is": "hardware FLOP per ray-step of the headline "
+                                             "workload"},
         "roofline_fp64": {"kernel": "k_evolve_f64<4>", "achieved": a64, "peak": PEAK_FP64_TFLOPS,
                           "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS},
     }
