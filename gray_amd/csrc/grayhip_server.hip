@@ -351,7 +351,7 @@ public:
 			C.n_over_dt = (double)C.sr.n / C.sr.dt;
 			C.dt_over_n = C.sr.dt / (double)C.sr.n;
 			C.dts = (float)C.sr.dts;
-			{
+			if (cp.key.n_freq > 0) {
 				const RtConst<SRV_NF> rc = make_rtconst<SRV_NF>(C);
 				for (int i = 0; i < SRV_NF; ++i) {
 					C.l2c[i] = rc.l2c[i];

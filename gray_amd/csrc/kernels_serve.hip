@@ -322,13 +322,16 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_kerr(SrvKerrArgs A,
 		const SrvClass &C = cls[c];
 		if (b < C.block0 || b >= C.block1)
 			continue;
-		switch (C.n_freq) {
-		case 0: kerr_lanes<0>(A, reqs, C, c); break;
-		case 1: kerr_lanes<1>(A, reqs, C, c); break;
-		case 2: kerr_lanes<2>(A, reqs, C, c); break;
-		case 3: kerr_lanes<3>(A, reqs, C, c); break;
-		default: kerr_lanes<4>(A, reqs, C, c); break;
-		}
+		/* two step widths: up to 2 channels and up to 4 (the extra
+		 * channels repeat channel 0 and are not reported; a class without
+		 * channels has fluid none, so its channels stay zero).  With three
+		 * or more instantiations inlined here the kernel needs 170
+		 * registers instead of 159-161: two waves per SIMD instead of
+		 * three */
+		if (C.n_freq <= 2)
+			kerr_lanes<2>(A, reqs, C, c);
+		else
+			kerr_lanes<4>(A, reqs, C, c);
 		return;
 	}
 }
