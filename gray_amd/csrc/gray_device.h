@@ -316,22 +316,34 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 	return acc;
 }
 
-/* f, l_i and their first derivatives at q in FP32: the quantities of the
- * first half of gr_rhs, from two reciprocal square roots and two
- * reciprocals (transcendentals issue at a quarter of the VALU rate) instead
- * of two square roots and four reciprocals; 2 / (rr + aa zz / rr) =
- * 2 rr / (rr^2 + aa zz).  d{j}_{g} is the j-derivative of g. */
-struct KsFieldsF32 {
-	float r, ir, f, dx_f, dy_f, dz_f;
-	float lx, dx_lx, dy_lx, dz_lx;
-	float ly, dx_ly, dy_ly, dz_ly;
-	float lz, dx_lz, dy_lz, dz_lz;
+/* The FP32 form's geodesic RHS: the Hamiltonian form of the same Kerr-Schild
+ * geodesics (sim-org/KS.cl:263-378 integrates u^mu), with the covariant
+ * momentum p_mu = g_{mu nu} u^nu as the state.  H = g^{mu nu} p_mu p_nu / 2,
+ * g^{mu nu} = eta^{mu nu} - f l^mu l^nu, l^mu = (-1, lx, ly, lz), L = l^mu p_mu:
+ *   dx^mu/dlambda = dH/dp_mu = eta^{mu nu} p_nu - f L l^mu,
+ *   dp_i/dlambda = -dH/dx^i = L (L d_i f / 2 + f d_i(l^nu p_nu)),
+ *   dp_t/dlambda = 0 (the metric is stationary),
+ * which needs f, l and first derivatives but no Christoffel contraction.
+ * With lx = (r x + a y) / (r^2 + a^2), ly = (r y - a x) / (r^2 + a^2),
+ * lz = z / r, A = x p_x + y p_y, B = y p_x - x p_y, S = lx p_x + ly p_y:
+ *   d_i(l^nu p_nu) = d_i r K1 + H_i,  K1 = (A - 2 r S) / (r^2 + a^2) - lz p_z / r,
+ *   H = ((r p_x - a p_y), (r p_y + a p_x)) / (r^2 + a^2), p_z / r;
+ *   d_i f = F1 d_i r (+ F2 for i = z), d_i r = c x_i (c = r / (2 r dd)
+ *   times r^2 for x, y and r^2 + a^2 for z),
+ * so dp_i = L (x_i c M + f H_i) (+ L^2 F2 / 2 for z), M = L F1 / 2 + f K1.
+ * r, 1/r and 1/(r dd) come from two reciprocal square roots and f from one
+ * reciprocal (transcendentals issue at a quarter of the VALU rate);
+ * 2 / (rr + aa zz / rr) = 2 rr / (rr^2 + aa zz).  Fills *aux (aux->lu = L =
+ * l_mu u^mu, as l is null). */
+struct HamF32 {
+	V4<float> u;             /* dx^mu / dlambda     */
+	float dpx, dpy, dpz;     /* dp_i / dlambda      */
 };
 
-__device__ __forceinline__ KsFieldsF32 ks_fields_f32(float a, const V4<float> &q)
+__device__ __forceinline__ HamF32 ks_ham_f32(float a, const V4<float> &q, const V4<float> &p,
+                                             KsAux<float> *aux = nullptr)
 {
 	typedef Mth<float> M;
-	KsFieldsF32 F;
 	const float aa = a * a;
 	const float zz = q.z * q.z;
 	const float kk = 0.5f * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
@@ -345,82 +357,41 @@ __device__ __forceinline__ KsFieldsF32 ks_fields_f32(float a, const V4<float> &q
 	const float two_rq = M::div(2.0f * rr, fmaT(aa, zz, rr * rr));
 	const float ss = rr + aa;
 	const float iss = M::div(1.0f, ss);
-	F.r = r;
-	F.ir = ir;
 
-	float tmp = half_rdd;
-	const float dz_r = (tmp * ss) * q.z;
-	tmp = tmp * rr;
-	const float dy_r = tmp * q.y;
-	const float dx_r = tmp * q.x;
-	tmp = two_rq;
-	const float tmp2 = fmaT(-(2.0f * rr), tmp, 3.0f);
-	F.f = tmp * r;
-	F.dx_f = (tmp * dx_r) * tmp2;
-	F.dy_f = (tmp * dy_r) * tmp2;
-	F.dz_f = tmp * fmaT(-((tmp * aa) * q.z), ir, dz_r * tmp2);
+	const float f = two_rq * r;
+	const float F1 = two_rq * fmaT(-(2.0f * rr), two_rq, 3.0f);
+	const float F2 = -two_rq * (((two_rq * aa) * q.z) * ir);
+	const float lx = iss * fmaT(q.y, a, q.x * r);
+	const float ly = iss * fmaT(-q.x, a, q.y * r);
+	const float lz = q.z * ir;
 
-	const float m2r = -2.0f * r;
-	const float issr = iss * r;
-	const float issa = iss * a;
-
-	F.lx = iss * fmaT(q.y, a, q.x * r);
-	tmp = iss * fmaT(m2r, F.lx, q.x);
-	F.dx_lx = fmaT(tmp, dx_r, issr);
-	F.dy_lx = fmaT(tmp, dy_r, issa);
-	F.dz_lx = tmp * dz_r;
-
-	F.ly = iss * fmaT(-q.x, a, q.y * r);
-	tmp = iss * fmaT(m2r, F.ly, q.y);
-	F.dx_ly = fmaT(tmp, dx_r, -issa);
-	F.dy_ly = fmaT(tmp, dy_r, issr);
-	F.dz_ly = tmp * dz_r;
-
-	F.lz = q.z * ir;
-	tmp = (-F.lz) * ir;
-	F.dx_lz = tmp * dx_r;
-	F.dy_lz = tmp * dy_r;
-	F.dz_lz = fmaT(tmp, dz_r, ir);
-	return F;
-}
-
-/* The FP32 form's geodesic RHS: the Hamiltonian form of the same Kerr-Schild
- * geodesics (sim-org/KS.cl:263-378 integrates u^mu), with the covariant
- * momentum p_mu = g_{mu nu} u^nu as the state.  H = g^{mu nu} p_mu p_nu / 2,
- * g^{mu nu} = eta^{mu nu} - f l^mu l^nu, l^mu = (-1, lx, ly, lz), L = l^mu p_mu:
- *   dx^mu/dlambda = dH/dp_mu = eta^{mu nu} p_nu - f L l^mu,
- *   dp_i/dlambda = -dH/dx^i = L (L d_i f / 2 + f (d_i l^nu) p_nu),
- *   dp_t/dlambda = 0 (the metric is stationary),
- * which needs f, l and their first derivatives but no Christoffel
- * contraction.  Fills *aux (aux->lu = L = l_mu u^mu, as l is null). */
-struct HamF32 {
-	V4<float> u;             /* dx^mu / dlambda     */
-	float dpx, dpy, dpz;     /* dp_i / dlambda      */
-};
-
-__device__ __forceinline__ HamF32 ks_ham_f32(float a, const V4<float> &q, const V4<float> &p,
-                                             KsAux<float> *aux = nullptr)
-{
-	const KsFieldsF32 F = ks_fields_f32(a, q);
-	const float L = fmaT(F.lz, p.z, fmaT(F.ly, p.y, fmaT(F.lx, p.x, -p.t)));
-	const float fL = F.f * L;
+	const float A = fmaT(q.y, p.y, q.x * p.x);
+	const float B = fmaT(q.y, p.x, -(q.x * p.y));
+	const float S = iss * fmaT(r, A, a * B);
+	const float lzpz = lz * p.z;
+	const float L = (S + lzpz) - p.t;
+	const float fL = f * L;
 	HamF32 h;
-	h.u = V4<float>{fL - p.t, fmaT(-fL, F.lx, p.x), fmaT(-fL, F.ly, p.y),
-	                fmaT(-fL, F.lz, p.z)};
+	h.u = V4<float>{fL - p.t, fmaT(-fL, lx, p.x), fmaT(-fL, ly, p.y), fmaT(-fL, lz, p.z)};
+
+	const float K1 = fmaT(-lzpz, ir, iss * fmaT(-2.0f * r, S, A));
 	const float hL = 0.5f * L;
-	const float gx = fmaT(F.dx_lz, p.z, fmaT(F.dx_ly, p.y, F.dx_lx * p.x));
-	const float gy = fmaT(F.dy_lz, p.z, fmaT(F.dy_ly, p.y, F.dy_lx * p.x));
-	const float gz = fmaT(F.dz_lz, p.z, fmaT(F.dz_ly, p.y, F.dz_lx * p.x));
-	h.dpx = L * fmaT(hL, F.dx_f, F.f * gx);
-	h.dpy = L * fmaT(hL, F.dy_f, F.f * gy);
-	h.dpz = L * fmaT(hL, F.dz_f, F.f * gz);
+	const float Mc = fmaT(hL, F1, f * K1);
+	const float Mxy = (half_rdd * rr) * Mc;
+	const float Mz = (half_rdd * ss) * Mc;
+	const float hx = iss * fmaT(r, p.x, -(a * p.y));
+	const float hy = iss * fmaT(r, p.y, a * p.x);
+	const float hz = p.z * ir;
+	h.dpx = L * fmaT(q.x, Mxy, f * hx);
+	h.dpy = L * fmaT(q.y, Mxy, f * hy);
+	h.dpz = L * fmaT(hL, F2, fmaT(q.z, Mz, f * hz));
 	if (aux) {
-		aux->r = F.r;
-		aux->ir = F.ir;
-		aux->f = F.f;
-		aux->lx = F.lx;
-		aux->ly = F.ly;
-		aux->lz = F.lz;
+		aux->r = r;
+		aux->ir = ir;
+		aux->f = f;
+		aux->lx = lx;
+		aux->ly = ly;
+		aux->lz = lz;
 		aux->lu = L;
 	}
 	return h;

@@ -111,3 +111,48 @@ def test_a_planted_sign_error_fails():
         scale = np.abs(dp_chain).max() + np.abs(du).max()
         worst = max(worst, np.abs(bad - dp_chain).max() / scale)
     assert worst > 1e-3
+
+
+def closed_form(a, q, p):
+    """The device's closed-form evaluation (gray_device.h ks_ham_f32), in
+    FP64: d_i(l.p) = d_i r K1 + H_i with K1 = (A - 2 r S_xy) / (r^2 + a^2) -
+    l_z p_z / r, d_i f = F1 d_i r (+ F2 for z), folded into dp_i."""
+    x, y, z = q[1:]
+    pt, px, py, pz = p
+    aa, zz = a * a, z * z
+    kk = 0.5 * ((x * x + y * y + zz) - aa)
+    dd = np.sqrt(kk * kk + aa * zz)
+    rr = dd + kk
+    r = np.sqrt(rr)
+    ir = 1 / r
+    half_rdd = 0.5 / (r * dd)
+    two_rq = 2 * rr / (rr * rr + aa * zz)
+    ss = rr + aa
+    iss = 1 / ss
+    f = two_rq * r
+    F1 = two_rq * (3 - 2 * rr * two_rq)
+    F2 = -two_rq * (two_rq * aa * z) * ir
+    lx, ly, lz = iss * (x * r + a * y), iss * (y * r - a * x), z * ir
+    A, B = x * px + y * py, y * px - x * py
+    s_xy = iss * (r * A + a * B)
+    L = s_xy + lz * pz - pt
+    fL = f * L
+    u = np.array([fL - pt, px - fL * lx, py - fL * ly, pz - fL * lz])
+    K1 = iss * (A - 2 * r * s_xy) - lz * pz * ir
+    M = 0.5 * L * F1 + f * K1
+    hx, hy, hz = iss * (r * px - a * py), iss * (r * py + a * px), pz * ir
+    cxy, cz = half_rdd * rr, half_rdd * ss
+    dp = np.array([0.0, L * (x * cxy * M + f * hx), L * (y * cxy * M + f * hy),
+                   L * (z * cz * M + f * hz + 0.5 * L * F2)])
+    return u, dp
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_closed_form_matches_hamiltons_equations(seed):
+    for a, q, u in events(40, seed):
+        p = O.down(a, q, u)
+        u_h, dp_h = ham_rhs(a, q, p)
+        u_c, dp_c = closed_form(a, q, p)
+        np.testing.assert_allclose(u_c, u_h, rtol=1e-13, atol=1e-13)
+        scale = np.abs(dp_h).max()
+        np.testing.assert_allclose(dp_c, dp_h, rtol=0, atol=1e-9 * scale)
