@@ -316,9 +316,10 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
 	return acc;
 }
 
-/* The FP32 form's geodesic RHS: the Hamiltonian form of the same Kerr-Schild
- * geodesics (sim-org/KS.cl:263-378 integrates u^mu), with the covariant
- * momentum p_mu = g_{mu nu} u^nu as the state.  H = g^{mu nu} p_mu p_nu / 2,
+/* The FP32 form's geodesic RHS (and its FP64 re-integration's): the
+ * Hamiltonian form of the same Kerr-Schild geodesics (sim-org/KS.cl:263-378
+ * integrates u^mu), with the covariant momentum p_mu = g_{mu nu} u^nu as
+ * the state.  H = g^{mu nu} p_mu p_nu / 2,
  * g^{mu nu} = eta^{mu nu} - f l^mu l^nu, l^mu = (-1, lx, ly, lz), L = l^mu p_mu:
  *   dx^mu/dlambda = dH/dp_mu = eta^{mu nu} p_nu - f L l^mu,
  *   dp_i/dlambda = -dH/dx^i = L (L d_i f / 2 + f d_i(l^nu p_nu)),
@@ -331,57 +332,63 @@ __device__ __forceinline__ V4<T> gr_rhs(T a, const V4<T> &q, const V4<T> &u,
  *   d_i f = F1 d_i r (+ F2 for i = z), d_i r = c x_i (c = r / (2 r dd)
  *   times r^2 for x, y and r^2 + a^2 for z),
  * so dp_i = L (x_i c M + f H_i) (+ L^2 F2 / 2 for z), M = L F1 / 2 + f K1.
- * r, 1/r and 1/(r dd) come from two reciprocal square roots and f from one
- * reciprocal (transcendentals issue at a quarter of the VALU rate);
- * 2 / (rr + aa zz / rr) = 2 rr / (rr^2 + aa zz).  Fills *aux (aux->lu = L =
- * l_mu u^mu, as l is null). */
-struct HamF32 {
-	V4<float> u;             /* dx^mu / dlambda     */
-	float dpx, dpy, dpz;     /* dp_i / dlambda      */
+ * r, 1/r and 1/(r dd) come from two reciprocal square roots and f and
+ * 1/(r^2 + a^2) from two reciprocals, R's rsq and rcp (HwF32: the hardware
+ * instructions, ~1 ulp; FP64: refined by Newton steps); 2 / (rr + aa zz /
+ * rr) = 2 rr / (rr^2 + aa zz).  Fills *aux (aux->lu = L = l_mu u^mu, as l is
+ * null). */
+struct HwF32 {
+	static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+	static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 
-__device__ __forceinline__ HamF32 ks_ham_f32(float a, const V4<float> &q, const V4<float> &p,
-                                             KsAux<float> *aux = nullptr)
+template <class T> struct Ham {
+	V4<T> u;                 /* dx^mu / dlambda     */
+	T dpx, dpy, dpz;         /* dp_i / dlambda      */
+};
+
+template <class T, class R>
+__device__ __forceinline__ Ham<T> ks_ham(T a, const V4<T> &q, const V4<T> &p,
+                                         KsAux<T> *aux = nullptr)
 {
-	typedef Mth<float> M;
-	const float aa = a * a;
-	const float zz = q.z * q.z;
-	const float kk = 0.5f * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
-	const float d2 = fmaT(aa, zz, kk * kk);
-	const float idd = __builtin_amdgcn_rsqf(d2);
-	const float dd = d2 * idd;
-	const float rr = dd + kk;
-	const float ir = __builtin_amdgcn_rsqf(rr);
-	const float r = rr * ir;
-	const float half_rdd = (0.5f * ir) * idd;
-	const float two_rq = M::div(2.0f * rr, fmaT(aa, zz, rr * rr));
-	const float ss = rr + aa;
-	const float iss = M::div(1.0f, ss);
+	const T aa = a * a;
+	const T zz = q.z * q.z;
+	const T kk = T(0.5) * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
+	const T d2 = fmaT(aa, zz, kk * kk);
+	const T idd = R::rsq(d2);
+	const T dd = d2 * idd;
+	const T rr = dd + kk;
+	const T ir = R::rsq(rr);
+	const T r = rr * ir;
+	const T half_rdd = (T(0.5) * ir) * idd;
+	const T two_rq = (T(2.0) * rr) * R::rcp(fmaT(aa, zz, rr * rr));
+	const T ss = rr + aa;
+	const T iss = R::rcp(ss);
 
-	const float f = two_rq * r;
-	const float F1 = two_rq * fmaT(-(2.0f * rr), two_rq, 3.0f);
-	const float F2 = -two_rq * (((two_rq * aa) * q.z) * ir);
-	const float lx = iss * fmaT(q.y, a, q.x * r);
-	const float ly = iss * fmaT(-q.x, a, q.y * r);
-	const float lz = q.z * ir;
+	const T f = two_rq * r;
+	const T F1 = two_rq * fmaT(-(T(2.0) * rr), two_rq, T(3.0));
+	const T F2 = -two_rq * (((two_rq * aa) * q.z) * ir);
+	const T lx = iss * fmaT(q.y, a, q.x * r);
+	const T ly = iss * fmaT(-q.x, a, q.y * r);
+	const T lz = q.z * ir;
 
-	const float A = fmaT(q.y, p.y, q.x * p.x);
-	const float B = fmaT(q.y, p.x, -(q.x * p.y));
-	const float S = iss * fmaT(r, A, a * B);
-	const float lzpz = lz * p.z;
-	const float L = (S + lzpz) - p.t;
-	const float fL = f * L;
-	HamF32 h;
-	h.u = V4<float>{fL - p.t, fmaT(-fL, lx, p.x), fmaT(-fL, ly, p.y), fmaT(-fL, lz, p.z)};
+	const T A = fmaT(q.y, p.y, q.x * p.x);
+	const T B = fmaT(q.y, p.x, -(q.x * p.y));
+	const T S = iss * fmaT(r, A, a * B);
+	const T lzpz = lz * p.z;
+	const T L = (S + lzpz) - p.t;
+	const T fL = f * L;
+	Ham<T> h;
+	h.u = V4<T>{fL - p.t, fmaT(-fL, lx, p.x), fmaT(-fL, ly, p.y), fmaT(-fL, lz, p.z)};
 
-	const float K1 = fmaT(-lzpz, ir, iss * fmaT(-2.0f * r, S, A));
-	const float hL = 0.5f * L;
-	const float Mc = fmaT(hL, F1, f * K1);
-	const float Mxy = (half_rdd * rr) * Mc;
-	const float Mz = (half_rdd * ss) * Mc;
-	const float hx = iss * fmaT(r, p.x, -(a * p.y));
-	const float hy = iss * fmaT(r, p.y, a * p.x);
-	const float hz = p.z * ir;
+	const T K1 = fmaT(-lzpz, ir, iss * fmaT(T(-2.0) * r, S, A));
+	const T hL = T(0.5) * L;
+	const T Mc = fmaT(hL, F1, f * K1);
+	const T Mxy = (half_rdd * rr) * Mc;
+	const T Mz = (half_rdd * ss) * Mc;
+	const T hx = iss * fmaT(r, p.x, -(a * p.y));
+	const T hy = iss * fmaT(r, p.y, a * p.x);
+	const T hz = p.z * ir;
 	h.dpx = L * fmaT(q.x, Mxy, f * hx);
 	h.dpy = L * fmaT(q.y, Mxy, f * hy);
 	h.dpz = L * fmaT(hL, F2, fmaT(q.z, Mz, f * hz));

@@ -42,6 +42,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 		s[i] = data[(size_t)i * n_rays + g];
 	if constexpr (NF > 0)
 		s[8 + NF] = s[8 + NF] / rc.d[0];
+	fb_lower(p.a, s);
 	const int64_t st0 = steps[g];
 	int64_t st = st0;
 	for (int d = d0; d < sr.n_dump; ++d) {
@@ -50,6 +51,7 @@ __global__ __launch_bounds__(256) void k_fallback_f64(RayParams p, StepRule sr,
 		if (retired)
 			break;
 	}
+	fb_raise(p.a, s);
 #pragma unroll
 	for (int i = 0; i < 8 + NF; ++i)
 		data[(size_t)i * n_rays + g] = s[i];

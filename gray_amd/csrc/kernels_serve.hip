@@ -269,17 +269,14 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			V4<double> qd = {x[0], x[1], x[2], x[3]}, ud = {x[4], x[5], x[6], x[7]};
 			finish_ray(R, tab, ray, x, NF, getuu(R.p.a, qd, ud), st, A.remaining, A.ctr, A.done);
 		} else {
-			/* the FP64 list's layout: geodesic, SRV_NF intensities (unused
-			 * ones 0), the shared depth T itself (converted to the channels'
-			 * depths only when the ray ends, so that epochs in FP64 round
-			 * nothing) */
+			/* the FP64 list's layout: t, x, y, z, p_mu (the Hamiltonian
+			 * state, as here), SRV_NF intensities (unused ones 0), the shared
+			 * depth T itself (converted to the channels' depths only when
+			 * the ray ends, so that epochs in FP64 round nothing) */
 			double x[SRV_NC];
 #pragma unroll
 			for (int i = 0; i < 8 + NF; ++i)
 				x[i] = (double)hi[i] + (double)lo[i];
-			const V4<double> ud = up(R.p.a, V4<double>{x[0], x[1], x[2], x[3]},
-			                         V4<double>{x[4], x[5], x[6], x[7]});
-			x[4] = ud.t; x[5] = ud.x; x[6] = ud.y; x[7] = ud.z;
 #pragma unroll
 			for (int i = 8 + NF; i < SRV_NC; ++i)
 				x[i] = 0.0;
@@ -389,6 +386,7 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A,
 				if (h < 0) {
 					/* non-finite FP32 state: the whole ray again in FP64 */
 					icond_ray<SRV_NF, RayParams>(reqs[tab].p, (size_t)ray, x);
+					fb_lower(reqs[tab].p.a, x);
 					d = 0;
 					h = 0;
 					st = 0;
@@ -415,18 +413,13 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A,
 			atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
 			break;
 		}
-		/* x[8 + SRV_NF] holds the channels' shared depth T (0 from the camera) */
+		/* x holds t, x, y, z, p_mu, the intensities and at 8 + SRV_NF the
+		 * channels' shared depth T (0 from the camera) */
 		const StepRule sr = R.sr;
-		V4<double> q = {x[0], x[1], x[2], x[3]};
-		V4<double> u = {x[4], x[5], x[6], x[7]};
-		const double xd = getdt(R.p, q, u, sr.dts) * ((double)sr.n / sr.dt);
-		int dh = (xd == xd) ? (int)xd : 0;
+		const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
+		const int dh = step_fb<SRV_NF, RayParams>(R.p, rc, sr, x, h);
 		bool fin = !dh;
 		if (!fin) {
-			if (dh > sr.n - h)
-				dh = sr.n - h;
-			const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
-			rk4_fb<SRV_NF, RayParams>(R.p, rc, x, (double)dh * sr.dt / (double)sr.n);
 			++st;
 			h += dh;
 			if (h >= sr.n) {
@@ -436,7 +429,7 @@ __global__ __launch_bounds__(SRV_BLOCK) void k_srv_fb64(SrvFbArgs A,
 		}
 		if (fin) {
 			atomicAdd(&A.ctr->steps[1], (unsigned long long)(st - st0));
-			const RtConst64<SRV_NF> rc = make_rtconst64<SRV_NF>(R.p);
+			fb_raise(R.p.a, x);
 			const double T = x[8 + SRV_NF];
 #pragma unroll
 			for (int i = 0; i < SRV_NF; ++i)

@@ -66,13 +66,21 @@ __device__ __forceinline__ RtConst64<NF> make_rtconst64(const P &p)
 	return r;
 }
 
-/* Density n and redshift factor sh of the disk at (q, u); n = 0 where
- * nothing is emitted.  K must come from gr_rhs at the same (q, u); its r is
- * the r of getrr, so the r_isco cut is the one of the step rule's
- * geometry. */
+/* k_mu = eta_mu nu u^nu + f (l.u) l_mu from the Kerr-Schild quantities of
+ * gr_rhs at (q, u) (the u^mu forms; the Hamiltonian forms carry k_mu). */
+__device__ __forceinline__ V4<double> cov_from_aux(const V4<double> &u, const KsAux<double> &K)
+{
+	const double flu = K.f * K.lu;
+	return V4<double>{flu - u.t, u.x + flu * K.lx, u.y + flu * K.ly, u.z + flu * K.lz};
+}
+
+/* Density n and redshift factor sh of the disk at q for the covariant
+ * photon momentum k; n = 0 where nothing is emitted.  K must come from the
+ * geodesic RHS at q; its r is the r of getrr, so the r_isco cut is the one
+ * of the step rule's geometry. */
 template <class P>
 __device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
-                                                const V4<double> &u, const KsAux<double> &K,
+                                                const V4<double> &k, const KsAux<double> &K,
                                                 double &n, double &sh)
 {
 	typedef MthFb M;
@@ -88,9 +96,7 @@ __device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
 			const double vv = (Om * Om * R2 - 1.0) + K.f * lv * lv;
 			if (vv < 0.0) {
 				const double ut = M::rsq(-vv);
-				const double flu = K.f * K.lu;
-				const double kx = u.x + flu * K.lx, ky = u.y + flu * K.ly;
-				const double kv = (flu - u.t) + Om * (q.x * ky - q.y * kx);
+				const double kv = k.t + Om * (q.x * k.y - q.y * k.x);
 				const double s_ = -kv * ut;
 				if (s_ > 0.0) {
 					sh = s_;
@@ -103,46 +109,19 @@ __device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
 	}
 }
 
-/* Fills dI[0..NF), dtau[0..NF) from the channels' depths tau. */
-template <int NF, class P>
-__device__ __forceinline__ void rt_rhs_disk64(const P &p, const RtConst64<NF> &rc,
-                                              const V4<double> &q, const V4<double> &u,
-                                              const KsAux<double> &K, const double *tau,
-                                              double *dI, double *dtau)
-{
-	typedef MthFb M;
-	double n, sh;
-	disk_emission64(p, q, u, K, n, sh);
-	if (n == 0.0) {
-#pragma unroll
-		for (int i = 0; i < NF; ++i) {
-			dI[i] = 0.0;
-			dtau[i] = 0.0;
-		}
-		return;
-	}
-	const double rsh = M::rsq(sh);
-	const double A = n * rsh * M::rcp(sh * sh + 1e-28);
-	const double Bc = n * (rsh * rsh) * rsh;
-#pragma unroll
-	for (int i = 0; i < NF; ++i) {
-		dI[i] = -(A * rc.c[i]) * M::exp_(-tau[i]);
-		dtau[i] = -(Bc * rc.d[i]);
-	}
-}
-
-/* The same with the channels' depths as one shared integral: the disk's
- * absorption is a fixed power of frequency, so tau_i = d_i T with
- * d_i = 0.1 nu_i^-5/2 (rc.d) and dT = -n sh^-3/2.  Fills dI[0..NF), dT. */
+/* The transfer RHS (sim-org/rt.cl:185-198) with the channels' depths as one
+ * shared integral: the disk's absorption is a fixed power of frequency, so
+ * tau_i = d_i T with d_i = 0.1 nu_i^-5/2 (rc.d) and dT = -n sh^-3/2.  k is
+ * the covariant photon momentum.  Fills dI[0..NF), dT. */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_disk64_shared(const P &p, const RtConst64<NF> &rc,
-                                                     const V4<double> &q, const V4<double> &u,
+                                                     const V4<double> &q, const V4<double> &k,
                                                      const KsAux<double> &K, double T,
                                                      double *dI, double &dT)
 {
 	typedef MthFb M;
 	double n, sh;
-	disk_emission64(p, q, u, K, n, sh);
+	disk_emission64(p, q, k, K, n, sh);
 	if (n == 0.0) {
 #pragma unroll
 		for (int i = 0; i < NF; ++i)

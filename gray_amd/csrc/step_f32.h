@@ -9,7 +9,7 @@
  * T = -int n shift^-3/2 dlambda.  The FP32 form integrates T once instead of
  * the n_freq depths.
  *
- * Its geodesic is the Hamiltonian form (ks_ham_f32): the state holds the
+ * Its geodesic is the Hamiltonian form (ks_ham): the state holds the
  * covariant momentum p_mu instead of u^mu, so a ray's state is t, x, y, z,
  * p_t, p_x, p_y, p_z, the n_freq intensities and T (ND32 reals); p_t is
  * conserved and not integrated.  load_state and store_state convert from
@@ -111,7 +111,7 @@ __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const
 	V4<float> q = {s[0], s[1], s[2], s[3]};
 	V4<float> k = {s[4], s[5], s[6], s[7]};
 	KsAux<float> K;
-	const HamF32 h = ks_ham_f32((float)p.a, q, k, NF > 0 ? &K : nullptr);
+	const Ham<float> h = ks_ham<float, HwF32>((float)p.a, q, k, NF > 0 ? &K : nullptr);
 	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
 	ds[4] = 0.f;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
 	if constexpr (NF > 0)

@@ -28,7 +28,8 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if constexpr (NF > 0)
-		rt_rhs_disk64_shared<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
+		rt_rhs_disk64_shared<NF, P>(p, rc, q, cov_from_aux(u, K), K, s[8 + NF], ds + 8,
+		                            ds[8 + NF]);
 }
 
 /* sim-org/RK4.cl:37-47: s + dt (k1 + 2 (k2 + k3) + k4) / 6, stages at

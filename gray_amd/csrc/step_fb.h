@@ -5,15 +5,19 @@
  * These rays are not part of the bit-exact form, and their dependency chain
  * of sequential steps is the critical path of a render's tail, so the step
  * is built for few instructions per step at FP64 accuracy:
- *   - divisions and square roots from the hardware FP64 reciprocal and
- *     reciprocal square root refined by two Newton steps (MthFb of
- *     rt_disk.h: about 1 ulp instead of correctly rounded);
- *   - the transfer RHS of rt_disk.h, the channels' depths as one shared
- *     integral T (the state is nd64 reals, as the plain form's);
- *   - the quantised step count as getdt (n / dt), one multiply;
+ *   - the Hamiltonian form of the geodesic, as the FP32 form's (ks_ham:
+ *     state t, x, y, z, p_mu, p_t conserved), with the reciprocals and
+ *     reciprocal square roots from the hardware FP64 instructions refined
+ *     by Newton steps (MthFb of rt_disk.h: about 1 ulp instead of
+ *     correctly rounded);
+ *   - the transfer RHS of rt_disk.h on k_mu = p_mu, the channels' depths as
+ *     one shared integral T (the state is nd64 reals, as the plain form's);
+ *   - the quantised step count as getdt (n / dt), one multiply, with u^mu
+ *     from the step's first RK4 stage;
  *   - the RK4 combination multiplied by 1/6.
  * The arithmetic is the same mathematics as sim-org/KS.cl:263-378,
- * rt.cl:185-198 and RK4.cl:37-47 in a different evaluation order.
+ * rt.cl:185-198 and RK4.cl:37-47 in a different form and evaluation order.
+ * fb_load / fb_store convert from and to the stored layout (u^mu) in FP64.
  */
 #pragma once
 
@@ -28,25 +32,29 @@ __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc, cons
                                        double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
-	V4<double> u = {s[4], s[5], s[6], s[7]};
+	V4<double> k = {s[4], s[5], s[6], s[7]};
 	KsAux<double> K;
-	V4<double> a = gr_rhs<double, MthFb>(p.a, q, u, NF > 0 ? &K : nullptr);
-	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
-	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
+	const Ham<double> h = ks_ham<double, MthFb>(p.a, q, k, NF > 0 ? &K : nullptr);
+	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
+	ds[4] = 0.0;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
 	if constexpr (NF > 0)
-		rt_rhs_disk64_shared<NF, P>(p, rc, q, u, K, s[8 + NF], ds + 8, ds[8 + NF]);
+		rt_rhs_disk64_shared<NF, P>(p, rc, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
-/* sim-org/RK4.cl:37-47 over rhs_fb. */
+/* sim-org/RK4.cl:37-47 over rhs_fb; k1 is rhs_fb at s, which the caller has
+ * evaluated. */
 template <int NF, class P>
-__device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, double *s, double dt)
+__device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, double *s,
+                                       const double *k1in, double dt)
 {
 	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
-	rhs_fb<NF, P>(p, rc, s, k1);
 #pragma unroll
-	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k1[i];
+	for (int i = 0; i < ND; ++i) {
+		k1[i] = k1in[i];
+		x[i] = s[i] + hdt * k1[i];
+	}
 	rhs_fb<NF, P>(p, rc, x, k23);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
@@ -63,7 +71,27 @@ __device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, doub
 		s[i] = s[i] + dt6 * (k1[i] + k[i]);
 }
 
-/* The quantised sub-steps of one interval (as interval_f64) over rk4_fb. */
+/* One quantised sub-step (sim-org/driver.cl:81-99) over rk4_fb: returns the
+ * units dh it advanced, 0 where the step rule stops the ray. */
+template <int NF, class P>
+__device__ __forceinline__ int step_fb(const P &p, const RtConst64<NF> &rc, const StepRule &sr,
+                                       double *s, int h)
+{
+	double k1[nd64<NF>()];
+	rhs_fb<NF, P>(p, rc, s, k1);
+	const V4<double> q = {s[0], s[1], s[2], s[3]};
+	const V4<double> u = {k1[0], k1[1], k1[2], k1[3]};
+	const double x = getdt(p, q, u, sr.dts) * ((double)sr.n / sr.dt);
+	int dh = (x == x) ? (int)x : 0;
+	if (!dh)
+		return 0;
+	if (dh > sr.n - h)
+		dh = sr.n - h;
+	rk4_fb<NF, P>(p, rc, s, k1, (double)dh * sr.dt / (double)sr.n);
+	return dh;
+}
+
+/* The quantised sub-steps of one interval (as interval_f64) over step_fb. */
 template <int NF, class P>
 __device__ __forceinline__ int64_t interval_fb(const P &p, const RtConst64<NF> &rc,
                                                const StepRule &sr, double *s, int h0,
@@ -72,22 +100,31 @@ __device__ __forceinline__ int64_t interval_fb(const P &p, const RtConst64<NF> &
 	int64_t st = 0;
 	int dh;
 	*retired = false;
-	const double n_over_dt = (double)sr.n / sr.dt;
 	for (int h = h0; h < sr.n; h += dh) {
-		V4<double> q = {s[0], s[1], s[2], s[3]};
-		V4<double> u = {s[4], s[5], s[6], s[7]};
-		double x = getdt(p, q, u, sr.dts) * n_over_dt;
-		dh = (x == x) ? (int)x : 0;
+		dh = step_fb<NF, P>(p, rc, sr, s, h);
 		if (!dh) {
 			*retired = true;
 			break;
 		}
-		if (dh > sr.n - h)
-			dh = sr.n - h;
-		rk4_fb<NF, P>(p, rc, s, (double)dh * sr.dt / (double)sr.n);
 		++st;
 	}
 	return st;
+}
+
+/* Stored geodesic (t, x, y, z, u^mu) -> the step's (t, x, y, z, p_mu), in
+ * place, and back. */
+__device__ __forceinline__ void fb_lower(double a, double *s)
+{
+	const V4<double> k = down(a, V4<double>{s[0], s[1], s[2], s[3]},
+	                          V4<double>{s[4], s[5], s[6], s[7]});
+	s[4] = k.t; s[5] = k.x; s[6] = k.y; s[7] = k.z;
+}
+
+__device__ __forceinline__ void fb_raise(double a, double *s)
+{
+	const V4<double> u = up(a, V4<double>{s[0], s[1], s[2], s[3]},
+	                        V4<double>{s[4], s[5], s[6], s[7]});
+	s[4] = u.t; s[5] = u.x; s[6] = u.y; s[7] = u.z;
 }
 
 } // namespace gray
