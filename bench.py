@@ -62,7 +62,7 @@ FLOP_PER_STEP = F_KS + f_rt(len(NUS))
 # operations per ray-step than the reference's u^mu form that FLOP_PER_STEP
 # counts; the plain FP64 form evaluates the reference's form, plus the
 # Newton steps of its correctly rounded divisions and square roots.
-HW_FLOP_POOL = 890
+HW_FLOP_POOL = 821
 HW_FLOP_FP64 = 2412
 HW_SOURCE = "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32/FP64 x 64 / ray-steps (profiles/)"
 PEAK_FP32_TFLOPS = 157.3
@@ -74,7 +74,7 @@ PEAK_HBM_GBPS = 8000.0
 # profiles/kerr_disk_1024x1024_kernels.md (k_pool_f32<4>, per pass) and
 # profiles/grid_bks128_512x512_kernels.md (k_grid_tiles_f32<0, float> and
 # k_grid_tiles_f64<0, double>).
-TRAFFIC_POOL_PASS = (73.05 + 85.62) * 2**20
+TRAFFIC_POOL_PASS = (57.80 + 82.81) * 2**20
 TRAFFIC_GRID_TILES = (526.93 + 21.78) * 2**20
 TRAFFIC_GRID_TILES64 = (4695.65 + 22.24) * 2**20
 TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/)"

@@ -1,4 +1,4 @@
-"""The FP32 form's geodesic equations (gray_device.h ks_ham_f32) against the
+"""The FP32 form's geodesic equations (gray_device.h ks_ham) against the
 oracle's Kerr-Schild geodesic RHS (sim-org/KS.cl:263-378).
 
 The FP32 form carries the covariant momentum p_mu = g_{mu nu} u^nu and
@@ -114,7 +114,7 @@ def test_a_planted_sign_error_fails():
 
 
 def closed_form(a, q, p):
-    """The device's closed-form evaluation (gray_device.h ks_ham_f32), in
+    """The device's closed-form evaluation (gray_device.h ks_ham), in
     FP64: d_i(l.p) = d_i r K1 + H_i with K1 = (A - 2 r S_xy) / (r^2 + a^2) -
     l_z p_z / r, d_i f = F1 d_i r (+ F2 for z), folded into dp_i."""
     x, y, z = q[1:]
