@@ -109,12 +109,64 @@ __device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
 	}
 }
 
+/* exp(-d_i T) of the channels at the shared depth T0 of an RK4 step's first
+ * stage; the later stages' factors follow from them (chan_exp). */
+template <int NF> struct ChanExp {
+	double E[NF > 0 ? NF : 1];
+	double T0;
+};
+
+template <int NF>
+__device__ __forceinline__ ChanExp<NF> chan_exp_init(const RtConst64<NF> &rc, double T0)
+{
+	ChanExp<NF> ce;
+	ce.T0 = T0;
+#pragma unroll
+	for (int i = 0; i < NF; ++i)
+		ce.E[i] = MthFb::exp_(-(rc.d[i] * T0));
+	return ce;
+}
+
+/* exp(-d_i T) for every channel at a stage's depth T: exp(-d_i T0) times
+ * exp(x_i), x_i = -d_i (T - T0), by its Taylor series to degree 5 where
+ * every |x_i| <= 2^-10 (truncation below 2^-69 relative; T moves little
+ * within a step), the exponential itself otherwise. */
+template <int NF>
+__device__ __forceinline__ void chan_exp(const RtConst64<NF> &rc, const ChanExp<NF> &ce, double T,
+                                         double *e)
+{
+	const double dT = T - ce.T0;
+	bool small = true;
+	double x[NF > 0 ? NF : 1];
+#pragma unroll
+	for (int i = 0; i < NF; ++i) {
+		x[i] = -(rc.d[i] * dT);
+		small = small && fabs(x[i]) <= 0x1p-10;
+	}
+	if (small) {
+#pragma unroll
+		for (int i = 0; i < NF; ++i) {
+			double t = __builtin_fma(x[i], 1.0 / 120.0, 1.0 / 24.0);
+			t = __builtin_fma(x[i], t, 1.0 / 6.0);
+			t = __builtin_fma(x[i], t, 0.5);
+			t = __builtin_fma(x[i], t, 1.0);
+			e[i] = ce.E[i] * __builtin_fma(x[i], t, 1.0);
+		}
+	} else {
+#pragma unroll
+		for (int i = 0; i < NF; ++i)
+			e[i] = MthFb::exp_(-(rc.d[i] * T));
+	}
+}
+
 /* The transfer RHS (sim-org/rt.cl:185-198) with the channels' depths as one
  * shared integral: the disk's absorption is a fixed power of frequency, so
  * tau_i = d_i T with d_i = 0.1 nu_i^-5/2 (rc.d) and dT = -n sh^-3/2.  k is
- * the covariant photon momentum.  Fills dI[0..NF), dT. */
+ * the covariant photon momentum, ce the channels' exponentials at the
+ * step's first stage.  Fills dI[0..NF), dT. */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_disk64_shared(const P &p, const RtConst64<NF> &rc,
+                                                     const ChanExp<NF> &ce,
                                                      const V4<double> &q, const V4<double> &k,
                                                      const KsAux<double> &K, double T,
                                                      double *dI, double &dT)
@@ -132,9 +184,11 @@ __device__ __forceinline__ void rt_rhs_disk64_shared(const P &p, const RtConst64
 	const double rsh = M::rsq(sh);
 	const double A = n * rsh * M::rcp(sh * sh + 1e-28);
 	dT = -(n * (rsh * rsh) * rsh);
+	double e[NF > 0 ? NF : 1];
+	chan_exp<NF>(rc, ce, T, e);
 #pragma unroll
 	for (int i = 0; i < NF; ++i)
-		dI[i] = -(A * rc.c[i]) * M::exp_(-(rc.d[i] * T));
+		dI[i] = -(A * rc.c[i]) * e[i];
 }
 
 } // namespace gray

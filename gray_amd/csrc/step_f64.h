@@ -18,8 +18,8 @@ template <int NF> constexpr int nd64() { return 8 + NF + (NF > 0 ? 1 : 0); }
 /* The geodesic in the written order with correctly rounded division and
  * square root (bit-exact); the transfer channels from rt_disk.h. */
 template <int NF, class P>
-__device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, const double *s,
-                                        double *ds)
+__device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc,
+                                        const ChanExp<NF> &ce, const double *s, double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
@@ -28,7 +28,7 @@ __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc, con
 	ds[0] = u.t; ds[1] = u.x; ds[2] = u.y; ds[3] = u.z;
 	ds[4] = a.t; ds[5] = a.x; ds[6] = a.y; ds[7] = a.z;
 	if constexpr (NF > 0)
-		rt_rhs_disk64_shared<NF, P>(p, rc, q, cov_from_aux(u, K), K, s[8 + NF], ds + 8,
+		rt_rhs_disk64_shared<NF, P>(p, rc, ce, q, cov_from_aux(u, K), K, s[8 + NF], ds + 8,
 		                            ds[8 + NF]);
 }
 
@@ -41,14 +41,15 @@ __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, dou
 	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
+	const ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
 
-	rhs_f64<NF, P>(p, rc, s, k1);
+	rhs_f64<NF, P>(p, rc, ce, s, k1);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = fmaT(hdt, k1[i], s[i]);
-	rhs_f64<NF, P>(p, rc, x, k23);
+	rhs_f64<NF, P>(p, rc, ce, x, k23);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = fmaT(hdt, k23[i], s[i]);
-	rhs_f64<NF, P>(p, rc, x, k);
+	rhs_f64<NF, P>(p, rc, ce, x, k);
 	/* k1 + 2 (k2 + k3) is formed before the last stage (same value), so
 	 * k1 and k2 + k3 are not both live across it */
 #pragma unroll
@@ -56,7 +57,7 @@ __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, dou
 		k1[i] = fmaT(2.0, k23[i] + k[i], k1[i]);
 		x[i] = fmaT(dt, k[i], s[i]);
 	}
-	rhs_f64<NF, P>(p, rc, x, k);
+	rhs_f64<NF, P>(p, rc, ce, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
 		s[i] = s[i] + div6(dt * (k1[i] + k[i]));

@@ -11,7 +11,8 @@
  *     by Newton steps (MthFb of rt_disk.h: about 1 ulp instead of
  *     correctly rounded);
  *   - the transfer RHS of rt_disk.h on k_mu = p_mu, the channels' depths as
- *     one shared integral T (the state is nd64 reals, as the plain form's);
+ *     one shared integral T (the state is nd64 reals, as the plain form's),
+ *     their exponentials from the step's first stage (chan_exp);
  *   - the quantised step count as getdt (n / dt), one multiply, with u^mu
  *     from the step's first RK4 stage;
  *   - the RK4 combination multiplied by 1/6.
@@ -28,8 +29,8 @@
 namespace gray {
 
 template <int NF, class P>
-__device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc, const double *s,
-                                       double *ds)
+__device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc,
+                                       const ChanExp<NF> &ce, const double *s, double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> k = {s[4], s[5], s[6], s[7]};
@@ -38,14 +39,15 @@ __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc, cons
 	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
 	ds[4] = 0.0;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
 	if constexpr (NF > 0)
-		rt_rhs_disk64_shared<NF, P>(p, rc, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
+		rt_rhs_disk64_shared<NF, P>(p, rc, ce, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
 
 /* sim-org/RK4.cl:37-47 over rhs_fb; k1 is rhs_fb at s, which the caller has
  * evaluated. */
 template <int NF, class P>
-__device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, double *s,
-                                       const double *k1in, double dt)
+__device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc,
+                                       const ChanExp<NF> &ce, double *s, const double *k1in,
+                                       double dt)
 {
 	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
@@ -55,16 +57,16 @@ __device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc, doub
 		k1[i] = k1in[i];
 		x[i] = s[i] + hdt * k1[i];
 	}
-	rhs_fb<NF, P>(p, rc, x, k23);
+	rhs_fb<NF, P>(p, rc, ce, x, k23);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) x[i] = s[i] + hdt * k23[i];
-	rhs_fb<NF, P>(p, rc, x, k);
+	rhs_fb<NF, P>(p, rc, ce, x, k);
 #pragma unroll
 	for (int i = 0; i < ND; ++i) {
 		k1[i] = k1[i] + 2.0 * (k23[i] + k[i]);
 		x[i] = s[i] + dt * k[i];
 	}
-	rhs_fb<NF, P>(p, rc, x, k);
+	rhs_fb<NF, P>(p, rc, ce, x, k);
 	const double dt6 = dt * (1.0 / 6.0);
 #pragma unroll
 	for (int i = 0; i < ND; ++i)
@@ -77,8 +79,10 @@ template <int NF, class P>
 __device__ __forceinline__ int step_fb(const P &p, const RtConst64<NF> &rc, const StepRule &sr,
                                        double *s, int h)
 {
-	double k1[nd64<NF>()];
-	rhs_fb<NF, P>(p, rc, s, k1);
+	constexpr int ND = nd64<NF>();
+	double k1[ND];
+	const ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
+	rhs_fb<NF, P>(p, rc, ce, s, k1);
 	const V4<double> q = {s[0], s[1], s[2], s[3]};
 	const V4<double> u = {k1[0], k1[1], k1[2], k1[3]};
 	const double x = getdt(p, q, u, sr.dts) * ((double)sr.n / sr.dt);
@@ -87,7 +91,7 @@ __device__ __forceinline__ int step_fb(const P &p, const RtConst64<NF> &rc, cons
 		return 0;
 	if (dh > sr.n - h)
 		dh = sr.n - h;
-	rk4_fb<NF, P>(p, rc, s, k1, (double)dh * sr.dt / (double)sr.n);
+	rk4_fb<NF, P>(p, rc, ce, s, k1, (double)dh * sr.dt / (double)sr.n);
 	return dh;
 }
 
