@@ -113,6 +113,38 @@ def test_fp32_form_image_error():
     assert abs(s1.sum() / s0.sum() - 1) < 5e-3
 
 
+def test_fp32_form_conserves_the_energy():
+    """The FP32 form integrates the Hamiltonian form of the geodesic, whose
+    state carries the covariant momentum: p_t (the photon's conserved
+    energy, the metric being stationary) is not integrated, so every ray,
+    pool or FP64 re-integration, ends with the k_t it started with, up to
+    the FP64 round trips u -> p -> u at load and store (a few ulp for a ray
+    that escapes; near the horizon, where a captured ray is handed to the
+    FP64 re-integration and back, the conversions lose more: u^t = f L - p_t
+    with f L >> p_t).  The u^mu form of the plain FP64 form lets k_t drift
+    by its truncation and rounding."""
+    with G.Render(48, 48, A, precision=G.FP32, nu=NUS) as r:
+        r.icond(64.0, 64.0, 1000.0, 60.0, 0.0)
+        k0 = r.read_covariant()[:, 4]
+        r.render(-32.0, 1024, 128)
+        d1, _ = r.read_f64()
+        k1 = r.read_covariant()[:, 4]
+        _, nfb = r.stats()
+    assert np.isfinite(k1).all() and nfb > 0
+    rel = np.abs(k1 - k0) / np.abs(k0)
+    esc = np.sqrt(np.sum(d1[:, 1:4] ** 2, axis=1)) > 999.0
+    assert esc.sum() > 1000
+    assert rel[esc].max() <= 1e-13, rel[esc].max()
+    assert rel.max() <= 1e-10, rel.max()
+    with G.Render(48, 48, A, precision=G.FP64, nu=NUS) as r:
+        r.icond(64.0, 64.0, 1000.0, 60.0, 0.0)
+        r.render(-32.0, 1024, 128)
+        k64 = r.read_covariant()[:, 4]
+    drift64 = np.abs(k64 - k0) / np.abs(k0)
+    print(f"k_t drift: FP32 form {rel[esc].max():.2e} escaping, {rel.max():.2e} all; "
+          f"plain FP64 form {drift64[esc].max():.2e} escaping, {drift64.max():.2e} all", flush=True)
+
+
 def test_fp32_per_launch_matches_pool():
     kw = dict(nu=NUS, precision=G.FP32)
     d1, _, s1, _ = gpu_render(A, 32, 32, 64.0, 64.0, 1000.0, 60.0, 0.0, **kw)
