@@ -172,6 +172,7 @@ __device__ __forceinline__ void gr_icond(T a, T r_obs, T si, T ci, T sj, T cj,
  * evaluation (used by the FP32 form's transfer). */
 template <class T> struct KsAux {
 	T r, ir, f, lx, ly, lz, lu;   /* ir = 1 / r */
+	T R2, zz, B;                  /* x^2 + y^2, z^2, y p_x - x p_y: ks_ham only */
 };
 
 /* sim-org/KS.cl:263-378: regrouped Cartesian Kerr-Schild geodesic RHS in
@@ -353,7 +354,8 @@ __device__ __forceinline__ Ham<T> ks_ham(T a, const V4<T> &q, const V4<T> &p,
 {
 	const T aa = a * a;
 	const T zz = q.z * q.z;
-	const T kk = T(0.5) * ((fmaT(q.y, q.y, q.x * q.x) + zz) - aa);
+	const T R2 = fmaT(q.y, q.y, q.x * q.x);
+	const T kk = T(0.5) * ((R2 + zz) - aa);
 	const T d2 = fmaT(aa, zz, kk * kk);
 	const T idd = R::rsq(d2);
 	const T dd = d2 * idd;
@@ -400,6 +402,9 @@ __device__ __forceinline__ Ham<T> ks_ham(T a, const V4<T> &q, const V4<T> &p,
 		aux->ly = ly;
 		aux->lz = lz;
 		aux->lu = L;
+		aux->R2 = R2;
+		aux->zz = zz;
+		aux->B = B;
 	}
 	return h;
 }

@@ -65,7 +65,7 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 	typedef Mth<float> M;
 	float n = 0.f, sh = 1.f;
 	if (p.fluid == GRAYHIP_FLUID_DISK) {
-		const float R2 = q.x * q.x + q.y * q.y;
+		const float R2 = K.R2;
 		if (K.r >= (float)p.r_isco && R2 > 0.f) {
 			/* r^(3/2) and r^(-3/2) from one reciprocal square root */
 			const float isr = __builtin_amdgcn_rsqf(K.r);
@@ -75,13 +75,14 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 			const float vv = (Om * Om * R2 - 1.f) + K.f * lv * lv;
 			if (vv < 0.f) {
 				const float ut = __builtin_amdgcn_rsqf(-vv);
-				const float kv = k.t + Om * (q.x * k.y - q.y * k.x);
+				/* k.v = k_t + Omega (x k_y - y k_x), x k_y - y k_x = -B */
+				const float kv = fmaf(-Om, K.B, k.t);
 				const float s_ = -kv * ut;
 				if (s_ > 0.f) {
 					sh = s_;
 					/* exp(-z^2 / (0.18 R^2)) as exp2 */
 					n = (K.ir * isr) *
-					    __builtin_amdgcn_exp2f((q.z * q.z) * __builtin_amdgcn_rcpf(R2) *
+					    __builtin_amdgcn_exp2f(K.zz * __builtin_amdgcn_rcpf(R2) *
 					                           (-1.4426950408889634f / 0.18f));
 				}
 			}
