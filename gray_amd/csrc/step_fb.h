@@ -8,7 +8,7 @@
  *   - the Hamiltonian form of the geodesic, as the FP32 form's (ks_ham:
  *     state t, x, y, z, p_mu, p_t conserved), with the reciprocals and
  *     reciprocal square roots from the hardware FP64 instructions refined
- *     by Newton steps (MthFb of rt_disk.h: about 1 ulp instead of
+ *     by a Newton step (MthFb of rt_disk.h: ~4e-15 relative instead of
  *     correctly rounded);
  *   - the transfer RHS of rt_disk.h on k_mu = p_mu, the channels' depths as
  *     one shared integral T (the state is nd64 reals, as the plain form's),
