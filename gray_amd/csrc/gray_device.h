@@ -346,6 +346,7 @@ struct HwF32 {
 template <class T> struct Ham {
 	V4<T> u;                 /* dx^mu / dlambda     */
 	T dpx, dpy, dpz;         /* dp_i / dlambda      */
+	T r;                     /* Kerr-Schild r of q  */
 };
 
 template <class T, class R>
@@ -394,6 +395,7 @@ __device__ __forceinline__ Ham<T> ks_ham(T a, const V4<T> &q, const V4<T> &p,
 	h.dpx = L * fmaT(q.x, Mxy, f * hx);
 	h.dpy = L * fmaT(q.y, Mxy, f * hy);
 	h.dpz = L * fmaT(hL, F2, fmaT(q.z, Mz, f * hz));
+	h.r = r;
 	if (aux) {
 		aux->r = r;
 		aux->ir = ir;
@@ -409,9 +411,33 @@ __device__ __forceinline__ Ham<T> ks_ham(T a, const V4<T> &q, const V4<T> &p,
 	return h;
 }
 
-/* sim-org/phys.cl:48-64 with the selectable escape rule.  P is any
- * parameter block with the fields a, r_h, eps_stop, escape, r_esc (the
- * render's RayParams, or per-lane parameters of a serving pool). */
+/* sim-org/phys.cl:48-64 with the selectable escape rule, for the
+ * Kerr-Schild r of q given.  P is any parameter block with the fields a,
+ * r_h, eps_stop, escape, r_esc (the render's RayParams, or per-lane
+ * parameters of a serving pool). */
+template <class T, class P>
+__device__ __forceinline__ T getdt_r(const P &p, T r, const V4<T> &q, const V4<T> &u, T dt)
+{
+	T eps = r - T(p.r_h);
+
+	if (eps < T(p.eps_stop))
+		return T(0.0);
+	if (p.escape == GRAYHIP_ESC_OUTWARD) {
+		T qu = (q.x * u.x + q.y * u.y) + q.z * u.z;
+		if (r > T(p.r_esc) && qu * dt > T(0.0))
+			return T(0.0);
+	} else {
+		T qq = (q.x * q.x + q.y * q.y) + q.z * q.z;
+		if (qq < T(0.0) && r > T(1e3))
+			return T(0.0);
+	}
+	T cap = (T(0.01) * eps) * eps;
+	if (fabs(dt) > cap)
+		return dt > T(0.0) ? cap : -cap;
+	return dt;
+}
+
+/* getdt_r with r from q (sim-org/phys.cl:48-64). */
 template <class T, class P>
 __device__ __forceinline__ T getdt(const P &p, const V4<T> &q,
                                    const V4<T> &u, T dt)
@@ -430,23 +456,7 @@ __device__ __forceinline__ T getdt(const P &p, const V4<T> &q,
 	} else {
 		r = Mth<T>::sqrt_(getrr(T(p.a), q));
 	}
-	T eps = r - T(p.r_h);
-
-	if (eps < T(p.eps_stop))
-		return T(0.0);
-	if (p.escape == GRAYHIP_ESC_OUTWARD) {
-		T qu = (q.x * u.x + q.y * u.y) + q.z * u.z;
-		if (r > T(p.r_esc) && qu * dt > T(0.0))
-			return T(0.0);
-	} else {
-		T qq = (q.x * q.x + q.y * q.y) + q.z * q.z;
-		if (qq < T(0.0) && r > T(1e3))
-			return T(0.0);
-	}
-	T cap = (T(0.01) * eps) * eps;
-	if (fabs(dt) > cap)
-		return dt > T(0.0) ? cap : -cap;
-	return dt;
+	return getdt_r(p, r, q, u, dt);
 }
 
 } // namespace gray

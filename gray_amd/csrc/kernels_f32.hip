@@ -114,12 +114,12 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 		if (g < 0)
 			break;
 
-		/* the first RK4 stage: its dx/dlambda is the u^mu of the step rule */
-		float k1[ND];
-		rhs_f32<NF>(pv, rc, hi, k1);
+		/* the first RK4 stage: its dx/dlambda and r are the step rule's */
+		float k1[ND], r1;
+		rhs_f32<NF>(pv, rc, hi, k1, &r1);
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {k1[0], k1[1], k1[2], k1[3]};
-		float gdt = getdt(pv, q, u, pc.dts);
+		float gdt = getdt_r(pv, r1, q, u, pc.dts);
 		double xd = (double)gdt * pc.n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		bool finish = (dh == 0);

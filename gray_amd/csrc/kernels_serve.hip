@@ -226,12 +226,12 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			atomicAdd(&A.ctr->steps[0], (unsigned long long)(st - st0));
 			break;
 		}
-		/* the first RK4 stage: its dx/dlambda is the u^mu of the step rule */
-		float k1[ND];
-		rhs_f32<NF>(P, rc, hi, k1);
+		/* the first RK4 stage: its dx/dlambda and r are the step rule's */
+		float k1[ND], r1;
+		rhs_f32<NF>(P, rc, hi, k1, &r1);
 		V4<float> q = {hi[0], hi[1], hi[2], hi[3]};
 		V4<float> u = {k1[0], k1[1], k1[2], k1[3]};
-		const float gdt = getdt(P, q, u, dts);
+		const float gdt = getdt_r(P, r1, q, u, dts);
 		const double xd = (double)gdt * n_over_dt;
 		int dh = (xd == xd) ? (int)xd : 0;
 		int state = 0;   /* 1 finished, 2 to FP64 here, 3 FP64 from the camera */

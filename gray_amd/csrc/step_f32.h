@@ -105,9 +105,11 @@ __device__ __forceinline__ void rt_rhs_f32(const P &p, const RtConst<NF> &rc, co
 		dI[i] = -A * __builtin_amdgcn_exp2f(fmaf(-rc.d2[i], T, rc.l2c[i]));
 }
 
+/* The FP32 state derivative; *r_out, when given, receives the Kerr-Schild
+ * r of the state's position (for the step rule). */
 template <int NF, class P>
 __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const float *s,
-                                        float *ds)
+                                        float *ds, float *r_out = nullptr)
 {
 	V4<float> q = {s[0], s[1], s[2], s[3]};
 	V4<float> k = {s[4], s[5], s[6], s[7]};
@@ -115,6 +117,8 @@ __device__ __forceinline__ void rhs_f32(const P &p, const RtConst<NF> &rc, const
 	const Ham<float> h = ks_ham<float, HwF32>((float)p.a, q, k, NF > 0 ? &K : nullptr);
 	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
 	ds[4] = 0.f;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
+	if (r_out)
+		*r_out = h.r;
 	if constexpr (NF > 0)
 		rt_rhs_f32<NF, P>(p, rc, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }

@@ -14,7 +14,7 @@
  *     one shared integral T (the state is nd64 reals, as the plain form's),
  *     their exponentials from the step's first stage (chan_exp);
  *   - the quantised step count as getdt (n / dt), one multiply, with u^mu
- *     from the step's first RK4 stage;
+ *     and r from the step's first RK4 stage;
  *   - the RK4 combination multiplied by 1/6.
  * The arithmetic is the same mathematics as sim-org/KS.cl:263-378,
  * rt.cl:185-198 and RK4.cl:37-47 in a different form and evaluation order.
@@ -30,7 +30,8 @@ namespace gray {
 
 template <int NF, class P>
 __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc,
-                                       const ChanExp<NF> &ce, const double *s, double *ds)
+                                       const ChanExp<NF> &ce, const double *s, double *ds,
+                                       double *r_out = nullptr)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> k = {s[4], s[5], s[6], s[7]};
@@ -38,6 +39,8 @@ __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc,
 	const Ham<double> h = ks_ham<double, MthFb>(p.a, q, k, NF > 0 ? &K : nullptr);
 	ds[0] = h.u.t; ds[1] = h.u.x; ds[2] = h.u.y; ds[3] = h.u.z;
 	ds[4] = 0.0;   ds[5] = h.dpx; ds[6] = h.dpy; ds[7] = h.dpz;
+	if (r_out)
+		*r_out = h.r;
 	if constexpr (NF > 0)
 		rt_rhs_disk64_shared<NF, P>(p, rc, ce, q, k, K, s[8 + NF], ds + 8, ds[8 + NF]);
 }
@@ -82,10 +85,11 @@ __device__ __forceinline__ int step_fb(const P &p, const RtConst64<NF> &rc, cons
 	constexpr int ND = nd64<NF>();
 	double k1[ND];
 	const ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
-	rhs_fb<NF, P>(p, rc, ce, s, k1);
+	double r1;
+	rhs_fb<NF, P>(p, rc, ce, s, k1, &r1);
 	const V4<double> q = {s[0], s[1], s[2], s[3]};
 	const V4<double> u = {k1[0], k1[1], k1[2], k1[3]};
-	const double x = getdt(p, q, u, sr.dts) * ((double)sr.n / sr.dt);
+	const double x = getdt_r(p, r1, q, u, sr.dts) * ((double)sr.n / sr.dt);
 	int dh = (x == x) ? (int)x : 0;
 	if (!dh)
 		return 0;
