@@ -62,7 +62,7 @@ FLOP_PER_STEP = F_KS + f_rt(len(NUS))
 # operations per ray-step than the reference's u^mu form that FLOP_PER_STEP
 # counts; the plain FP64 form evaluates the reference's form, plus the
 # Newton steps of its correctly rounded divisions and square roots.
-HW_FLOP_POOL = 809
+HW_FLOP_POOL = 799
 HW_FLOP_FP64 = 2176
 HW_SOURCE = "rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32/FP64 x 64 / ray-steps (profiles/)"
 PEAK_FP32_TFLOPS = 157.3
