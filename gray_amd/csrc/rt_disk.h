@@ -111,20 +111,44 @@ __device__ __forceinline__ void disk_emission64(const P &p, const V4<double> &q,
 }
 
 /* exp(-d_i T) of the channels at the shared depth T0 of an RK4 step's first
- * stage; the later stages' factors follow from them (chan_exp). */
+ * stage; the later stages' factors follow from them (chan_exp).  E is
+ * filled at once (chan_exp_init) or at the first stage whose emission is
+ * non-zero (chan_exp_defer; same values), which spares the exponentials to
+ * steps that emit nothing, such as a ray's approach to the horizon inside
+ * r_isco. */
 template <int NF> struct ChanExp {
 	double E[NF > 0 ? NF : 1];
 	double T0;
+	bool ready;   /* E holds exp(-d_i T0) */
 };
+
+template <int NF>
+__device__ __forceinline__ void chan_exp_fill(const RtConst64<NF> &rc, ChanExp<NF> &ce)
+{
+#pragma unroll
+	for (int i = 0; i < NF; ++i)
+		ce.E[i] = MthFb::exp_(-(rc.d[i] * ce.T0));
+	ce.ready = true;
+}
 
 template <int NF>
 __device__ __forceinline__ ChanExp<NF> chan_exp_init(const RtConst64<NF> &rc, double T0)
 {
 	ChanExp<NF> ce;
 	ce.T0 = T0;
+	chan_exp_fill<NF>(rc, ce);
+	return ce;
+}
+
+template <int NF>
+__device__ __forceinline__ ChanExp<NF> chan_exp_defer(double T0)
+{
+	ChanExp<NF> ce;
+	ce.T0 = T0;
 #pragma unroll
-	for (int i = 0; i < NF; ++i)
-		ce.E[i] = MthFb::exp_(-(rc.d[i] * T0));
+	for (int i = 0; i < (NF > 0 ? NF : 1); ++i)
+		ce.E[i] = 0.0;
+	ce.ready = false;
 	return ce;
 }
 
@@ -133,9 +157,11 @@ __device__ __forceinline__ ChanExp<NF> chan_exp_init(const RtConst64<NF> &rc, do
  * every |x_i| <= 2^-10 (truncation below 2^-69 relative; T moves little
  * within a step), the exponential itself otherwise. */
 template <int NF>
-__device__ __forceinline__ void chan_exp(const RtConst64<NF> &rc, const ChanExp<NF> &ce, double T,
+__device__ __forceinline__ void chan_exp(const RtConst64<NF> &rc, ChanExp<NF> &ce, double T,
                                          double *e)
 {
+	if (!ce.ready)
+		chan_exp_fill<NF>(rc, ce);
 	const double dT = T - ce.T0;
 	bool small = true;
 	double x[NF > 0 ? NF : 1];
@@ -167,7 +193,7 @@ __device__ __forceinline__ void chan_exp(const RtConst64<NF> &rc, const ChanExp<
  * step's first stage.  Fills dI[0..NF), dT. */
 template <int NF, class P>
 __device__ __forceinline__ void rt_rhs_disk64_shared(const P &p, const RtConst64<NF> &rc,
-                                                     const ChanExp<NF> &ce,
+                                                     ChanExp<NF> &ce,
                                                      const V4<double> &q, const V4<double> &k,
                                                      const KsAux<double> &K, double T,
                                                      double *dI, double &dT)

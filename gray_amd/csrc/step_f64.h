@@ -19,7 +19,7 @@ template <int NF> constexpr int nd64() { return 8 + NF + (NF > 0 ? 1 : 0); }
  * square root (bit-exact); the transfer channels from rt_disk.h. */
 template <int NF, class P>
 __device__ __forceinline__ void rhs_f64(const P &p, const RtConst64<NF> &rc,
-                                        const ChanExp<NF> &ce, const double *s, double *ds)
+                                        ChanExp<NF> &ce, const double *s, double *ds)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
 	V4<double> u = {s[4], s[5], s[6], s[7]};
@@ -41,7 +41,7 @@ __device__ __forceinline__ void rk4_f64(const P &p, const RtConst64<NF> &rc, dou
 	constexpr int ND = nd64<NF>();
 	double k1[ND], k[ND], x[ND], k23[ND];
 	const double hdt = 0.5 * dt;
-	const ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
+	ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
 
 	rhs_f64<NF, P>(p, rc, ce, s, k1);
 #pragma unroll

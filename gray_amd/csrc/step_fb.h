@@ -12,7 +12,8 @@
  *     correctly rounded);
  *   - the transfer RHS of rt_disk.h on k_mu = p_mu, the channels' depths as
  *     one shared integral T (the state is nd64 reals, as the plain form's),
- *     their exponentials from the step's first stage (chan_exp);
+ *     their exponentials from the step's first stage (chan_exp), formed
+ *     only in steps that emit;
  *   - the quantised step count as getdt (n / dt), one multiply, with u^mu
  *     and r from the step's first RK4 stage;
  *   - the RK4 combination multiplied by 1/6.
@@ -30,7 +31,7 @@ namespace gray {
 
 template <int NF, class P>
 __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc,
-                                       const ChanExp<NF> &ce, const double *s, double *ds,
+                                       ChanExp<NF> &ce, const double *s, double *ds,
                                        double *r_out = nullptr)
 {
 	V4<double> q = {s[0], s[1], s[2], s[3]};
@@ -49,7 +50,7 @@ __device__ __forceinline__ void rhs_fb(const P &p, const RtConst64<NF> &rc,
  * evaluated. */
 template <int NF, class P>
 __device__ __forceinline__ void rk4_fb(const P &p, const RtConst64<NF> &rc,
-                                       const ChanExp<NF> &ce, double *s, const double *k1in,
+                                       ChanExp<NF> &ce, double *s, const double *k1in,
                                        double dt)
 {
 	constexpr int ND = nd64<NF>();
@@ -84,7 +85,7 @@ __device__ __forceinline__ int step_fb(const P &p, const RtConst64<NF> &rc, cons
 {
 	constexpr int ND = nd64<NF>();
 	double k1[ND];
-	const ChanExp<NF> ce = chan_exp_init<NF>(rc, NF > 0 ? s[ND - 1] : 0.0);
+	ChanExp<NF> ce = chan_exp_defer<NF>(NF > 0 ? s[ND - 1] : 0.0);
 	double r1;
 	rhs_fb<NF, P>(p, rc, ce, s, k1, &r1);
 	const V4<double> q = {s[0], s[1], s[2], s[3]};
