@@ -13,8 +13,8 @@
  * position at r ~ 1000 and the slowly varying momentum accurate far below
  * one FP32 ulp per step.  Camera initial data come from the FP64 icond
  * kernel; the quantised step schedule (sim-org/driver.cl:81-99) is computed
- * in FP64.  A ray whose step falls below 1e-4 M (horizon approach) is handed
- * to the FP64 kernel, which continues it from its compensated state
+ * in FP64.  A ray whose step falls below FB_STEP_MIN (horizon approach) is
+ * handed to the FP64 kernel, which continues it from its compensated state
  * (hi + lo promoted to double); a ray whose FP32 state turns non-finite is
  * redone in FP64 from the state the pass started from.
  */
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_pool_f32(RayParams p, StepRule sr,
 			if (dh > sr.n - h)
 				dh = sr.n - h;
 			double hd = (double)dh * pc.dt_over_n;
-			if (fabs(hd) < 1e-4) {
+			if (fabs(hd) < FB_STEP_MIN) {
 				/* step collapse: FP64 re-integration from here */
 				store_state<NF>(p, data, n_rays, (size_t)g, hi, lo);
 				steps[g] += dst;

@@ -241,7 +241,7 @@ __device__ __forceinline__ void kerr_lanes(const SrvKerrArgs &A,
 			if (dh > sr.n - h)
 				dh = sr.n - h;
 			const double hd = (double)dh * dt_over_n;
-			if (fabs(hd) < 1e-4) {
+			if (fabs(hd) < FB_STEP_MIN) {
 				state = 2;
 			} else {
 				rk4_f32c<NF>(P, rc, hi, lo, k1, (float)hd);

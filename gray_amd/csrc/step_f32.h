@@ -28,6 +28,16 @@ template <int NF> constexpr int nd32() { return 8 + NF + (NF > 0 ? 1 : 0); }
 /* Index of p_t in it (constant along the ray). */
 constexpr int PT32 = 4;
 
+/* A ray whose quantised step falls below this affine length (M) goes to the
+ * FP64 re-integration.  Under the step rule's cap 0.01 (r - r_H)^2 that is
+ * r - r_H < 0.022, the last stretch of a plunge before the stop at
+ * r - r_H < 0.01: a step there still moves the position by tens of FP32
+ * ulps, which the compensated state keeps.  With a threshold of 1e-4 (r -
+ * r_H < 0.1) the rays winding at the prograde photon orbit of a = 0.9375
+ * (r - r_H = 0.088) were re-integrated in FP64 too, and their chain of
+ * ~1e5 sequential FP64 steps outlasted the pool. */
+constexpr double FB_STEP_MIN = 5e-6;
+
 /* Per-channel constants of the synthetic emissivity in base 2:
  * l2c_i = log2 nu_i^-1/2 and d2_i = 0.1 nu_i^-5/2 log2(e), so that a
  * channel's factor nu_i^-1/2 exp(-tau_i) = 2^(l2c_i - d2_i T) is one FMA
