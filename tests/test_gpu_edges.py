@@ -50,6 +50,19 @@ def test_spins(a):
     check_fp64(*both(a, 12, 10, 24.0, 20.0, 60.0, j_obs=20.0, nu=(2.67,)), 1)
 
 
+@pytest.mark.parametrize("a", [0.0, -0.7, 0.999])
+def test_fp32_form_spins(a):
+    """The FP32 form (pool, compensated state, FP64 re-integration below a
+    5e-6 M step) against the oracle at the same spins, on a plane across
+    the shadow's edge: for a = 0.999 the prograde photon orbit lies at
+    r - r_H ~ 0.03, where the rays winding on it stay in the FP32 pool."""
+    ref, got = both(a, 16, 16, 16.0, 16.0, 60.0, j_obs=20.0, nu=(2.67,), precision=G.FP32)
+    (d0, _, s0), (d1, _, s1) = ref, got
+    l1 = np.abs(d1[:, 8] - d0[:, 8]).sum() / np.abs(d0[:, 8]).sum()
+    assert l1 <= 1e-4, l1
+    assert abs(s1.sum() / s0.sum() - 1) < 5e-3, (s1.sum(), s0.sum())
+
+
 @pytest.mark.parametrize("i_obs", [0.0, 90.0, 180.0])
 def test_camera_orientations(i_obs):
     check_fp64(*both(0.9, 10, 10, 20.0, 20.0, i_obs, nu=(1.0,)), 1)
